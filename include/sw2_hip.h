@@ -1,0 +1,246 @@
+/*
+ * sw2_hip.h -- C-ABI of the MI355X (gfx950) shallow-water step library.
+ *
+ * The library is a drop-in for the time loop of SW2 (Shallow Worlds 2): the
+ * same three entry points that SW2's driver calls (main/Main.c:72-78), the
+ * same field layout (sppde map with a halo of 2, x fastest), and the same
+ * case-file input.  Everything that SW2 computes per step inside loop()
+ * (core.c:134-338) runs on the GPU; set-up, case parsing, frame IO and the
+ * diagnostics printed on the host stay in host C.
+ *
+ * Three layers are exported:
+ *   1. the coarse SW2 boundary: initialize / loop / destroy (sw.h:222-224);
+ *   2. the sppde helpers that SW2's driver links against (sppde/sppde.h);
+ *   3. the sw2_* extension API: building a world from parameters instead of a
+ *      case file, device-resident sessions (the step loop without host IO),
+ *      diagnostics, and the batched / served ensemble API.
+ *
+ * All signatures use plain C types and pointers.  Fields are host arrays of
+ * M->S doubles laid out as sppde does (ac(v,i,j,M) below).
+ */
+#ifndef SW2_HIP_H
+#define SW2_HIP_H
+
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------
+ * sppde-compatible map and field access (sppde/sppde.h:46-67, 131-142).
+ * Only the single-map case is meaningful here: one rank owns the whole
+ * domain, x is periodic, y is walled (channel).
+ * ------------------------------------------------------------------------- */
+#define SW2_MAXP 1 /* one GPU, one map */
+
+typedef struct {
+    int sh;        /* halo width (2) */
+    int nd;        /* number of dimensions (2) */
+    int l0[4];     /* first owned index per axis (1-based axes) */
+    int l1[4];     /* last owned index per axis */
+    int gl0[4];    /* global first index */
+    int gl1[4];    /* global last index */
+    int s[4];      /* extent per axis including halos */
+    int S;         /* number of doubles in a field */
+    int gpc[4];    /* processors per axis (all 1) */
+    int pc[4];     /* my processor coordinates (all 1) */
+    int nb_e, nb_w, nb_n, nb_s; /* neighbours (unused: single map) */
+    int bc0[4];    /* do I own the initial boundary of axis a (0 if periodic) */
+    int bc1[4];    /* do I own the final boundary of axis a (0 if periodic) */
+    int per[4];    /* periodicity per axis */
+} map;
+
+/* position of (i,j) in a field: x fastest, halo of sh cells on each side */
+#define LP(i, j, M) ((i) - ((M)->l0[1]) + (M)->sh + ((j) - ((M)->l0[2]) + (M)->sh) * ((M)->s[1]))
+#define ac(v, i, j, M) *((v) + LP(i, j, M))
+
+#ifndef MAXOUTLEN
+#define MAXOUTLEN 1000
+#endif
+
+/* ---------------------------------------------------------------------------
+ * The world (sw.h:22-95).  Field names, types and meaning are SW2's so that
+ * a driver written against sw.h compiles unchanged against this header.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    int sch;                 /* limiter: 0 MUSCL .. 9 Lax-Wendroff (calc.c:54-63) */
+    int polar;               /* 0: channel (the only geometry on the GPU path) */
+    int N;                   /* number of steps */
+    int loadFrom;            /* frame to start from (0: none) */
+    int saveEvery;           /* frame period in steps */
+    int IteInfo;             /* diagnostics period in steps */
+    int prn;                 /* diagnostics on this step */
+    double t0, t1, Dt;
+    double Courant_max;
+    int nx, ny, nz;
+    double lon0, lat0;       /* radians */
+    double lon1, lat1;       /* radians */
+    map m;
+    map *M;
+
+    double Omega;
+    double rE, rP;
+    double epsilon2;
+    double gE;
+    double depth;
+    double sigma_dis;
+    double tracer_dis;
+
+    double beta_sponge;
+    double sigma_sponge;
+    double tau_sponge;
+
+    double *windx;
+    double *windy;
+    double *u, *v, *eta, *hB;
+    double *tracer, *perturbs;
+    double *xe, *xc, *xv;
+    double *yn, *yc, *yv;
+    double *Dxe, *Dxc, *Dxv, *Dyn, *Dyc, *Dyv;
+    double *ge, *gn, *gc, *gv, *fe, *fn, *fc, *fv;
+
+    double nu2, nu4, nu6;
+
+    int reftype;
+    char refname[MAXOUTLEN];
+    double *reftable;
+    int reftable_nr, reftable_nc;
+
+    int geoeq;
+
+    int num_Gaussians;
+    double *Gaussians;       /* num_Gaussians x 12, row major (perturbs.c:55-56) */
+
+    int num_vortices;
+    double *vortices;        /* num_vortices x 7 */
+} sw;
+
+#define N_COLS_GAUSS 12
+#define N_COLS_VORTICES 7
+
+/* ---------------------------------------------------------------------------
+ * 1. Coarse SW2 boundary.
+ * ------------------------------------------------------------------------- */
+
+/* Replaces initialize() of init.c:23 (sw.h:222).  Reads <bname>.sw (m4
+ * define/include subset, <expr> evaluation, tag order of init.c:48-229) and
+ * builds the world.  npx*npy must be 1 unless force1proc==1 (one GPU owns the
+ * whole domain).  Returns 0; fatal errors abort like SW2's CRASH. */
+int initialize(int npx, int npy, int force1proc, char *bname, int copyinput, sw *SW);
+
+/* Replaces loop() of core.c:22 (sw.h:223).  Runs steps nstart+1..N on the
+ * GPU with SW2's per-step order and halo semantics, writes the frames and the
+ * ENERGY / stats lines SW2 writes, and leaves eta, u, v, tracer and perturbs
+ * (halos included) holding the final state in host memory. */
+void loop(char *bname, char *output_folder, sw *SW);
+
+/* Replaces destroy() of finish.c:21 (sw.h:224). */
+void destroy(sw *SW);
+
+/* ---------------------------------------------------------------------------
+ * 2. sppde helpers the SW2 driver calls (sppde/sppde.h:21-40, 353-362).
+ *    Single process: quants() is 1 and pprintf writes to stdout.
+ * ------------------------------------------------------------------------- */
+void checkr(int r, char *s);
+void crash(int line, char *file, char *fmt, ...);
+#define CRASH(...) crash(__LINE__, __FILE__, __VA_ARGS__)
+void init_pprintf(int mode, int flushall, char *output_folder);
+void end_pprintf(void);
+int pprintf(char *fmt, ...);
+int quants(void);
+int quisoc(void);
+double *dmem(map *M);
+void cr_reset(void);
+void cr_start(char *s, int level);
+void cr_end(char *s, int level);
+double cr_time(char *s, int level);
+void cr_info(void);
+
+/* ---------------------------------------------------------------------------
+ * 3. sw2_* extension API (no counterpart in SW2 unless stated).
+ * ------------------------------------------------------------------------- */
+
+/* Case parameters: what initialize() reads from a case file (init.c:48-229),
+ * angles in degrees as written in the file.  Tables are borrowed pointers. */
+typedef struct {
+    int sch, polar;
+    double lon0_deg, lon1_deg, lat0_deg, lat1_deg;
+    int nx, ny;
+    double t0, t1, Dt, CFL;
+    double Tau;              /* >1e10 disables the Rayleigh drag */
+    double hypernu[3];       /* as written: nu2, nu4 (entered positive), nu6 */
+    int IteInfo, SaveEvery, LoadFrom, reftype;
+    double Omega, rE, rP, gE, depth;
+    int geoeq;
+    int num_vortices;  const double *vortices;   /* rows x 7 */
+    int num_Gaussians; const double *Gaussians;  /* rows x 12 */
+    double TracerDis;
+    int wind_rows;     const double *wind;       /* rows x 2: lat_deg, u */
+} sw2_params;
+
+/* Build a world from parameters (the part of initialize() after parsing).
+ * Returns 0 or a negative error (see sw2_last_error). */
+int sw2_init_params(const sw2_params *P, sw *SW);
+
+/* Parse a case file into parameters; tables are allocated and owned by the
+ * library until sw2_free_params.  Returns 0 or a negative error. */
+int sw2_read_case(const char *fname, sw2_params *P);
+void sw2_free_params(sw2_params *P);
+
+/* Evaluate one case-file <expr> (tinyexpr grammar of sppde_tinyexpr.c). */
+double sw2_eval_expr(const char *expr, int *error);
+
+/* Single-map halo update of one field (sppde_hupdate.c:274-363). */
+void halo_update(double *x, map *M);
+
+/* Host versions kept for post-processing (postproc/Postproc.c:166,189). */
+void do_channel(sw *SW);
+void calc_energy_balance(double *ke, double *ape, sw *SW);
+
+/* Mass diagnostic (no counterpart in SW2): sum of H*Dxc*Dyc over the
+ * interior in SW2's loop order. */
+double sw2_mass(sw *SW);
+
+/* Frame IO in SW2's MAGICSW2 format (io.c:27-171). */
+void save_binary_sw(int frame, int nstep, double timedouble, char *bname, char *output_folder, sw *SW);
+int load_binary_sw(int frame, int *nstep, double *timedouble, char *bname, char *binaries_folder, sw *SW);
+
+/* Forms of the device step. */
+#define SW2_FORM_FP64  0   /* plain FP64, same results as SW2's CPU path */
+#define SW2_FORM_MIXED 1   /* FP32 anomalies about an FP64 balanced mean */
+
+/* A device session owns the device copy of one world.  Creating it uploads
+ * eta, u, v, tracer, perturbs, hB, windx, windy (halos included) and the
+ * per-row metric tables; steps then run without host traffic except for the
+ * events SW2 handles on the host (Gaussian injection, vortices). */
+typedef struct sw2_session sw2_session;
+
+int sw2_session_create(sw *SW, int form, sw2_session **out);
+/* Advance nsteps steps.  n0 is the index of the step before the first one
+ * (SW2's nstart), t the time before it; AB restarts at step n0+1 as SW2's
+ * loop does (core.c:166 n-nstart). */
+int sw2_session_begin(sw2_session *s, int nstart, double t);
+int sw2_session_step(sw2_session *s, int nsteps);
+/* Advance nsteps steps and return their device time in ms, measured with
+ * HIP events on the session's stream. */
+int sw2_session_time(sw2_session *s, int nsteps, double *ms);
+/* Copy eta, u, v, tracer, perturbs (halos included) back into SW's arrays. */
+int sw2_session_download(sw2_session *s, sw *SW);
+/* First step whose CFL or stats guard failed (0 if none) and its message. */
+int sw2_session_failed_step(sw2_session *s, char *msg, int msglen);
+/* Dt_CFL of the most recent step (compute_Dt, calc.c:701-738). */
+int sw2_session_last_cfl(sw2_session *s, double *dtcfl);
+/* Current step index and time. */
+int sw2_session_state(sw2_session *s, int *n, double *t);
+void sw2_session_destroy(sw2_session *s);
+
+/* Last error message of this thread's most recent failing sw2_* call. */
+const char *sw2_last_error(void);
+/* Library build string (architecture, flags). */
+const char *sw2_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SW2_HIP_H */

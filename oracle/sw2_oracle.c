@@ -1,0 +1,940 @@
+/*
+ * sw2_oracle.c -- CPU restatement of SW2's FP64 time step, single map.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the GPU library:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * it.  The product path never links or calls it.
+ *
+ * It restates, in plain C with the same expression association, SW2's loop()
+ * (core.c:22-373) on one rank: every field is a full (nx+4)*(ny+4) array with
+ * a halo of 2, halo updates follow the single-map rules of
+ * sppde/sppde_hupdate.c:274-363 (periodic copies in y as well as x, corners
+ * included), and do_channel (calc.c:600-649) rewrites interior columns only.
+ * Reads that SW2 makes from never-written heap memory (laplacian.c:117-127)
+ * read zero here.  Built with -O3 -ffp-contract=off and no -march flag, so no
+ * multiply-add is contracted, as in SW2's build (CMakeLists.txt:15).
+ *
+ * Pinning: tests/test_oracle.py checks this file against the energy series
+ * SW2 ships (examples/JupiterEnergyTest/enou.txt), closed forms of the
+ * geometry, limiters and Laplacians, and invariants of the step.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "../include/sw2_hip.h"
+
+#define OPI (4.0 * atan(1.0))            /* sw.h:16 */
+#define DEG2RAD(a) ((a) * (OPI / 180.0)) /* sw.h:108 */
+#define RAD2DEG(b) ((b) * (180.0 / OPI)) /* sw.h:109 */
+#define OMIN(a, b) (((a) < (b)) ? (a) : (b)) /* sw.h:104 */
+#define OMAX(a, b) (((a) > (b)) ? (a) : (b)) /* sw.h:105 */
+
+typedef struct orc {
+    int nx, ny, sx, S, sch;
+    double lon0, lon1, lat0, lat1;
+    double Dt, Courant, sigma_dis, tracer_dis, nu2, nu4, nu6;
+    double Omega, rE, rP, eps2, gE, depth;
+    int IteInfo, N;
+    /* fields, full haloed arrays */
+    double *eta, *u, *v, *tracer, *perturbs, *hB, *windx, *windy;
+    /* per-row geometry (index j+1, j=-1..ny+2); SW2 stores them as 2-D
+     * arrays whose value depends on j only (coords.c:53-59, init.c:285-296) */
+    double *dxe, *dxc, *dxv, *dyn, *dyc, *dyv, *ge, *gn, *gc, *gv, *fe, *fn, *fc, *fv;
+    /* AB history and work arrays (core.c:55-81) */
+    double *Dtr[3], *Trab;
+    double *Deta[3], *Etab;
+    double *Du[3], *Dv[3], *Duab, *Dvab, *Duadv, *Dvadv, *Dupres, *Dvpres;
+    double *Lu, *Lv, *LLu, *LLv, *LLLu, *LLLv, *ze, *di, *le, *lle, *llle, *psi, *xi, *pf;
+    int nGauss, nVort;
+    double *Gauss, *Vort;
+    int nstart, n;
+    double t;
+    int failed_step;
+    char failmsg[256];
+    int prn;
+    double last_dtcfl;
+    int tracer_active;
+} orc;
+
+#define IX(i, j) ((i) + 1 + ((j) + 1) * o->sx)
+#define RJ(j) ((j) + 1)
+#define F(a, i, j) (a)[IX(i, j)]
+
+/* ---------------- geometry (sw.h:113-204, coords.c:36-59, tools.c:256-273) */
+static double lin(int i, double x0, double x1, int n) {
+    return (x0 + (double)i * (x1 - x0) / ((double)n));
+}
+#define LATN(j) lin((j), o->lat0, o->lat1, o->ny)
+#define LATC(j) ((LATN(j) + LATN((j)-1)) / 2.)
+#define LONE(i) lin((i), o->lon0, o->lon1, o->nx)
+#define LONC(i) ((LONE(i) + LONE((i)-1)) / 2.)
+#define RZ(lat) ((o->rE) / sqrt(1.0 + (tan(lat) * tan(lat)) / (o->eps2)))
+#define RM(lat) ((o->rE) / (o->eps2) * ((RZ(lat) / ((o->rE) * cos(lat))) * (RZ(lat) / ((o->rE) * cos(lat))) * (RZ(lat) / ((o->rE) * cos(lat)))))
+#define L2PC(lat) atan(1.0 / (o->eps2) * tan(lat))
+#define RPC(p) ((o->rE) * (o->rP) / sqrt((o->rE * sin(p)) * (o->rE * sin(p)) + (o->rP * cos(p)) * (o->rP * cos(p))))
+#define G0(p) ((o->gE + ((o->Omega) * (o->Omega)) * (o->rE)) * (((o->rE) / (RPC(p))) * ((o->rE) / (RPC(p)))))
+#define GEFF(p) (G0(p) - ((o->Omega) * (o->Omega)) * RPC(p) * cos(p))
+#define FCOR(lat) (2.0 * (o->Omega) * sin(lat))
+
+static double *zalloc(size_t n) {
+    double *p = (double *)calloc(n, sizeof(double));
+    if (!p) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+    return p;
+}
+
+static void geometry(orc *o) {
+    int ny = o->ny;
+    double dlon = (o->lon1 - o->lon0) / ((double)o->nx);
+    double dlat = (o->lat1 - o->lat0) / ((double)o->ny);
+    for (int j = -1; j <= ny + 2; j++) {
+        int r = RJ(j);
+        o->dxe[r] = dlon * RZ(LATC(j));
+        o->dxc[r] = dlon * RZ(LATC(j));
+        o->dxv[r] = dlon * RZ(LATN(j));
+        o->dyn[r] = dlat * RM(LATC(j));
+        o->dyc[r] = dlat * RM(LATN(j - 1));
+        o->dyv[r] = dlat * RM(LATC(j));
+        o->ge[r] = GEFF(L2PC(LATC(j)));
+        o->gn[r] = GEFF(L2PC(LATN(j)));
+        o->gc[r] = GEFF(L2PC(LATC(j)));
+        o->gv[r] = GEFF(L2PC(LATN(j)));
+        o->fe[r] = FCOR(LATC(j));
+        o->fn[r] = FCOR(LATN(j));
+        o->fc[r] = FCOR(LATC(j));
+        o->fv[r] = FCOR(LATN(j));
+    }
+}
+
+/* ---------------- halo rules (sppde_hupdate.c:274-363; calc.c:600-649) --- */
+static void halo_update_o(orc *o, double *x) {
+    int nx = o->nx, ny = o->ny;
+    /* y: interior columns only, periodic copy even though y is walled */
+    for (int i = 1; i <= nx; i++) {
+        F(x, i, -1) = F(x, i, ny - 1);
+        F(x, i, 0) = F(x, i, ny);
+        F(x, i, ny + 1) = F(x, i, 1);
+        F(x, i, ny + 2) = F(x, i, 2);
+    }
+    /* x: all rows, corners included */
+    for (int j = -1; j <= ny + 2; j++) {
+        F(x, -1, j) = F(x, nx - 1, j);
+        F(x, 0, j) = F(x, nx, j);
+        F(x, nx + 1, j) = F(x, 1, j);
+        F(x, nx + 2, j) = F(x, 2, j);
+    }
+}
+
+static void do_channel_o(orc *o) {
+    int nx = o->nx, ny = o->ny;
+    for (int i = 1; i <= nx; i++) { /* forsouth, j = 1 */
+        int j = 1;
+        F(o->v, i, j - 1) = 0;
+        F(o->v, i, j - 2) = 0;
+        F(o->windy, i, j - 1) = 0;
+        F(o->windy, i, j - 2) = 0;
+        F(o->u, i, j - 1) = F(o->u, i, j);
+        F(o->u, i, j - 2) = F(o->u, i, j - 1);
+        F(o->windx, i, j - 1) = F(o->windx, i, j);
+        F(o->windx, i, j - 2) = F(o->windx, i, j - 1);
+    }
+    for (int i = 1; i <= nx; i++) { /* fornorth, j = ny */
+        int j = ny;
+        F(o->v, i, j) = 0;
+        F(o->v, i, j + 1) = 0;
+        F(o->v, i, j + 2) = 0;
+        F(o->windy, i, j) = 0;
+        F(o->windy, i, j + 1) = 0;
+        F(o->windy, i, j + 2) = 0;
+        F(o->u, i, j + 1) = F(o->u, i, j);
+        F(o->u, i, j + 2) = F(o->u, i, j + 1);
+        F(o->windx, i, j + 1) = F(o->windx, i, j);
+        F(o->windx, i, j + 2) = F(o->windx, i, j + 1);
+    }
+}
+
+/* ---------------- limiters (calc.c:54-63) -------------------------------- */
+static inline double psi_muscl(double r) { return (OMAX(0, OMIN(OMIN((1 + r) / 2.0, 2.0 * r), 2.0))); }
+static inline double psi_superbee(double r) { return (OMAX(OMAX(0, OMIN(2.0 * r, 1.0)), OMAX(0, OMIN(r, 2.0)))); }
+static inline double psi_minmod(double r) { return (OMIN(r, 1)); }
+static inline double psi_vanleer(double r) { return ((r + fabs(r)) / (1.0 + r)); }
+static inline double psi_koren(double r) { return (OMAX(0, OMIN(2.0 * r, OMIN((1 + 2.0 * r) / 3.0, 2.0)))); }
+static inline double psi_ospre(double r) { return ((1.5 * ((r * r) + r)) / ((r * r) + r + 1.0)); }
+static inline double psi_vanalbada1(double r) { return (((r * r) + r) / ((r * r) + 1.0)); }
+static inline double psi_umist(double r) { return (OMAX(0, OMIN(OMIN(2.0, 2.0 * r), OMIN(0.25 + 0.75 * r, 0.75 + 0.25 * r)))); }
+static inline double psi_upwind(double r) { (void)r; return (0); }
+static inline double psi_laxwendroff(double r) { (void)r; return (1.0); }
+typedef double (*psifn)(double);
+
+double orc_psi(int sch, double r) {
+    switch (sch) {
+    case 0: return psi_muscl(r);
+    case 1: return psi_superbee(r);
+    case 2: return psi_minmod(r);
+    case 3: return psi_vanleer(r);
+    case 4: return psi_koren(r);
+    case 5: return psi_ospre(r);
+    case 6: return psi_vanalbada1(r);
+    case 7: return psi_umist(r);
+    case 8: return psi_upwind(r);
+    case 9: return psi_laxwendroff(r);
+    }
+    return NAN;
+}
+
+/* guarded slope ratios (calc.c:43-44) and limited face values (calc.c:48-49) */
+static inline double r2rP(double Bprev, double Bcurr, double Bpost) {
+    return (((Bpost - Bcurr) != 0) ? ((Bcurr - Bprev) / (Bpost - Bcurr)) : 0);
+}
+static inline double r2rM(double Bcurr, double Bpost, double Bnext) {
+    return (((Bpost - Bcurr) != 0) ? ((Bnext - Bpost) / (Bpost - Bcurr)) : 0);
+}
+#define B2BPG(G, B1, B2, rP, CP) (B1 + G(rP) * 0.5 * (1 - CP) * (B2 - B1))
+#define B2BMG(G, B1, B2, rM, CM) (B2 - G(rM) * 0.5 * (1 + CM) * (B2 - B1))
+
+/* ---------------- the flux-limited bodies (calc.c:70-237) ---------------- */
+#define HH(i, j) ((F(o->eta, i, j)) + (o->depth) - (F(o->hB, i, j)))
+#define UW(i, j) (F(o->u, i, j) + F(o->windx, i, j))
+#define VW(i, j) (F(o->v, i, j) + F(o->windy, i, j))
+#define TR(i, j) F(o->tracer, i, j)
+
+/* Generic conservative face-flux body over a centred scalar B (BODYH and
+ * BODYTRACER share it, calc.c:70-163) */
+#define FLUXBODY(G, B, i, j, Dt)                                                       \
+    double dxe = o->dxe[RJ(j)], dyn = o->dyn[RJ(j)];                                   \
+    double CwP = 0.5 * (UW(i - 1, j) + fabs(UW(i - 1, j))) * Dt / dxe;                 \
+    double CwM = 0.5 * (UW(i - 1, j) - fabs(UW(i - 1, j))) * Dt / dxe;                 \
+    double CeP = 0.5 * (UW(i, j) + fabs(UW(i, j))) * Dt / dxe;                         \
+    double CeM = 0.5 * (UW(i, j) - fabs(UW(i, j))) * Dt / dxe;                         \
+    double CsP = 0.5 * (VW(i, j - 1) + fabs(VW(i, j - 1))) * Dt / dyn;                 \
+    double CsM = 0.5 * (VW(i, j - 1) - fabs(VW(i, j - 1))) * Dt / dyn;                 \
+    double CnP = 0.5 * (VW(i, j) + fabs(VW(i, j))) * Dt / dyn;                         \
+    double CnM = 0.5 * (VW(i, j) - fabs(VW(i, j))) * Dt / dyn;                         \
+    double rxPprev = r2rP(B(i - 2, j), B(i - 1, j), B(i, j));                          \
+    double rxMprev = r2rM(B(i - 1, j), B(i, j), B(i + 1, j));                          \
+    double rxP = r2rP(B(i - 1, j), B(i, j), B(i + 1, j));                              \
+    double rxM = r2rM(B(i, j), B(i + 1, j), B(i + 2, j));                              \
+    double ryPprev = r2rP(B(i, j - 2), B(i, j - 1), B(i, j));                          \
+    double ryMprev = r2rM(B(i, j - 1), B(i, j), B(i, j + 1));                          \
+    double ryP = r2rP(B(i, j - 1), B(i, j), B(i, j + 1));                              \
+    double ryM = r2rM(B(i, j), B(i, j + 1), B(i, j + 2));                              \
+    double BwP = B2BPG(G, B(i - 1, j), B(i, j), rxPprev, CwP);                         \
+    double BwM = B2BMG(G, B(i - 1, j), B(i, j), rxMprev, CwM);                         \
+    double BeP = B2BPG(G, B(i, j), B(i + 1, j), rxP, CeP);                             \
+    double BeM = B2BMG(G, B(i, j), B(i + 1, j), rxM, CeM);                             \
+    double BsP = B2BPG(G, B(i, j - 1), B(i, j), ryPprev, CsP);                         \
+    double BsM = B2BMG(G, B(i, j - 1), B(i, j), ryMprev, CsM);                         \
+    double BnP = B2BPG(G, B(i, j), B(i, j + 1), ryP, CnP);                             \
+    double BnM = B2BMG(G, B(i, j), B(i, j + 1), ryM, CnM);                             \
+    double flux = CwP * BwP + CwM * BwM - CeP * BeP - CeM * BeM + CsP * BsP + CsM * BsM - CnP * BnP - CnM * BnM;
+
+static inline __attribute__((always_inline)) void body_h(orc *o, psifn G, double *out, double Dt) {
+    for (int j = 1; j <= o->ny; j++)
+        for (int i = 1; i <= o->nx; i++) {
+            FLUXBODY(G, HH, i, j, Dt)
+            double d = flux;
+            d -= 2.0 * o->Dt * (o->sigma_dis) * F(o->eta, i, j);
+            F(out, i, j) = d;
+        }
+}
+
+static inline __attribute__((always_inline)) void body_tracer(orc *o, psifn G, double *out, double Dt) {
+    for (int j = 1; j <= o->ny; j++)
+        for (int i = 1; i <= o->nx; i++) {
+            FLUXBODY(G, TR, i, j, Dt)
+            double pv2 = Dt * TR(i, j) * (UW(i, j) - UW(i - 1, j)) / dxe;
+            pv2 += Dt * TR(i, j) * (VW(i, j) - VW(i, j - 1)) / dyn;
+            double d = flux + pv2;
+            d -= 2.0 * o->Dt * (o->tracer_dis) * TR(i, j);
+            F(out, i, j) = d;
+        }
+}
+
+/* BODYU (calc.c:168-200) */
+static inline __attribute__((always_inline)) void body_u(orc *o, psifn G, double *out, double Dt) {
+    for (int j = 1; j <= o->ny; j++) {
+        double dxc = o->dxc[RJ(j)], dyv = o->dyv[RJ(j)], dxe = o->dxe[RJ(j)];
+        for (int i = 1; i <= o->nx; i++) {
+            double CwP = 0.5 * (0.5 * (UW(i - 1, j) + fabs(UW(i - 1, j))) + 0.5 * (UW(i, j) + fabs(UW(i, j)))) * Dt / dxc;
+            double CwM = 0.5 * (0.5 * (UW(i - 1, j) - fabs(UW(i - 1, j))) + 0.5 * (UW(i, j) - fabs(UW(i, j)))) * Dt / dxc;
+            double CeP = 0.5 * (0.5 * (UW(i, j) + fabs(UW(i, j))) + 0.5 * (UW(i + 1, j) + fabs(UW(i + 1, j)))) * Dt / dxc;
+            double CeM = 0.5 * (0.5 * (UW(i, j) - fabs(UW(i, j))) + 0.5 * (UW(i + 1, j) - fabs(UW(i + 1, j)))) * Dt / dxc;
+            double CsP = 0.5 * (0.5 * (VW(i, j - 1) + fabs(VW(i, j - 1))) + 0.5 * (VW(i + 1, j - 1) + fabs(VW(i + 1, j - 1)))) * Dt / dyv;
+            double CsM = 0.5 * (0.5 * (VW(i, j - 1) - fabs(VW(i, j - 1))) + 0.5 * (VW(i + 1, j - 1) - fabs(VW(i + 1, j - 1)))) * Dt / dyv;
+            double CnP = 0.5 * (0.5 * (VW(i, j) + fabs(VW(i, j))) + 0.5 * (VW(i + 1, j) + fabs(VW(i + 1, j)))) * Dt / dyv;
+            double CnM = 0.5 * (0.5 * (VW(i, j) - fabs(VW(i, j))) + 0.5 * (VW(i + 1, j) - fabs(VW(i + 1, j)))) * Dt / dyv;
+            double rxPprev = r2rP(UW(i - 2, j), UW(i - 1, j), UW(i, j));
+            double rxMprev = r2rM(UW(i - 1, j), UW(i, j), UW(i + 1, j));
+            double rxP = r2rP(UW(i - 1, j), UW(i, j), UW(i + 1, j));
+            double rxM = r2rM(UW(i, j), UW(i + 1, j), UW(i + 2, j));
+            double ryPprev = r2rP(UW(i, j - 2), UW(i, j - 1), UW(i, j));
+            double ryMprev = r2rM(UW(i, j - 1), UW(i, j), UW(i, j + 1));
+            double ryP = r2rP(UW(i, j - 1), UW(i, j), UW(i, j + 1));
+            double ryM = r2rM(UW(i, j), UW(i, j + 1), UW(i, j + 2));
+            double BwP = B2BPG(G, UW(i - 1, j), UW(i, j), rxPprev, CwP);
+            double BwM = B2BMG(G, UW(i - 1, j), UW(i, j), rxMprev, CwM);
+            double BeP = B2BPG(G, UW(i, j), UW(i + 1, j), rxP, CeP);
+            double BeM = B2BMG(G, UW(i, j), UW(i + 1, j), rxM, CeM);
+            double BsP = B2BPG(G, UW(i, j - 1), UW(i, j), ryPprev, CsP);
+            double BsM = B2BMG(G, UW(i, j - 1), UW(i, j), ryMprev, CsM);
+            double BnP = B2BPG(G, UW(i, j), UW(i, j + 1), ryP, CnP);
+            double BnM = B2BMG(G, UW(i, j), UW(i, j + 1), ryM, CnM);
+            double p1 = CwP * BwP + CwM * BwM - CeP * BeP - CeM * BeM + CsP * BsP + CsM * BsM - CnP * BnP - CnM * BnM;
+            double p2 = Dt * (UW(i, j) * (UW(i + 1, j) - UW(i - 1, j)) / (dxe + o->dxe[RJ(j)]) +
+                              UW(i, j) * ((VW(i, j) + VW(i + 1, j)) / 2. - (VW(i, j - 1) + VW(i + 1, j - 1)) / 2.) / dyv);
+            F(out, i, j) = p1 + p2;
+        }
+    }
+}
+
+/* BODYV (calc.c:205-237) */
+static inline __attribute__((always_inline)) void body_v(orc *o, psifn G, double *out, double Dt) {
+    for (int j = 1; j <= o->ny; j++) {
+        double dxv = o->dxv[RJ(j)], dyc = o->dyc[RJ(j)];
+        double dyn2 = o->dyn[RJ(j)] + o->dyn[RJ(j + 1)];
+        for (int i = 1; i <= o->nx; i++) {
+            double CwP = 0.5 * (0.5 * (UW(i - 1, j + 1) + fabs(UW(i - 1, j + 1))) + 0.5 * (UW(i - 1, j) + fabs(UW(i - 1, j)))) * Dt / dxv;
+            double CwM = 0.5 * (0.5 * (UW(i - 1, j + 1) - fabs(UW(i - 1, j + 1))) + 0.5 * (UW(i - 1, j) - fabs(UW(i - 1, j)))) * Dt / dxv;
+            double CeP = 0.5 * (0.5 * (UW(i, j + 1) + fabs(UW(i, j + 1))) + 0.5 * (UW(i, j) + fabs(UW(i, j)))) * Dt / dxv;
+            double CeM = 0.5 * (0.5 * (UW(i, j + 1) - fabs(UW(i, j + 1))) + 0.5 * (UW(i, j) - fabs(UW(i, j)))) * Dt / dxv;
+            double CsP = 0.5 * (0.5 * (VW(i, j - 1) + fabs(VW(i, j - 1))) + 0.5 * (VW(i, j) + fabs(VW(i, j)))) * Dt / dyc;
+            double CsM = 0.5 * (0.5 * (VW(i, j - 1) - fabs(VW(i, j - 1))) + 0.5 * (VW(i, j) - fabs(VW(i, j)))) * Dt / dyc;
+            double CnP = 0.5 * (0.5 * (VW(i, j) + fabs(VW(i, j))) + 0.5 * (VW(i, j + 1) + fabs(VW(i, j + 1)))) * Dt / dyc;
+            double CnM = 0.5 * (0.5 * (VW(i, j) - fabs(VW(i, j))) + 0.5 * (VW(i, j + 1) - fabs(VW(i, j + 1)))) * Dt / dyc;
+            double rxPprev = r2rP(VW(i - 2, j), VW(i - 1, j), VW(i, j));
+            double rxMprev = r2rM(VW(i - 1, j), VW(i, j), VW(i + 1, j));
+            double rxP = r2rP(VW(i - 1, j), VW(i, j), VW(i + 1, j));
+            double rxM = r2rM(VW(i, j), VW(i + 1, j), VW(i + 2, j));
+            double ryPprev = r2rP(VW(i, j - 2), VW(i, j - 1), VW(i, j));
+            double ryMprev = r2rM(VW(i, j - 1), VW(i, j), VW(i, j + 1));
+            double ryP = r2rP(VW(i, j - 1), VW(i, j), VW(i, j + 1));
+            double ryM = r2rM(VW(i, j), VW(i, j + 1), VW(i, j + 2));
+            double BwP = B2BPG(G, VW(i - 1, j), VW(i, j), rxPprev, CwP);
+            double BwM = B2BMG(G, VW(i - 1, j), VW(i, j), rxMprev, CwM);
+            double BeP = B2BPG(G, VW(i, j), VW(i + 1, j), rxP, CeP);
+            double BeM = B2BMG(G, VW(i, j), VW(i + 1, j), rxM, CeM);
+            double BsP = B2BPG(G, VW(i, j - 1), VW(i, j), ryPprev, CsP);
+            double BsM = B2BMG(G, VW(i, j - 1), VW(i, j), ryMprev, CsM);
+            double BnP = B2BPG(G, VW(i, j), VW(i, j + 1), ryP, CnP);
+            double BnM = B2BMG(G, VW(i, j), VW(i, j + 1), ryM, CnM);
+            double p1 = CwP * BwP + CwM * BwM - CeP * BeP - CeM * BeM + CsP * BsP + CsM * BsM - CnP * BnP - CnM * BnM;
+            double p2 = Dt * (VW(i, j) * ((UW(i, j) + UW(i, j + 1)) / 2. - (UW(i - 1, j) + UW(i - 1, j + 1)) / 2.) / dxv +
+                              VW(i, j) * (VW(i, j + 1) - VW(i, j - 1)) / dyn2);
+            F(out, i, j) = p1 + p2;
+        }
+    }
+}
+
+#define DISPATCH(fn, ...)                                                    \
+    switch (o->sch) {                                                        \
+    case 0: fn(o, psi_muscl, __VA_ARGS__); break;                            \
+    case 1: fn(o, psi_superbee, __VA_ARGS__); break;                         \
+    case 2: fn(o, psi_minmod, __VA_ARGS__); break;                           \
+    case 3: fn(o, psi_vanleer, __VA_ARGS__); break;                          \
+    case 4: fn(o, psi_koren, __VA_ARGS__); break;                            \
+    case 5: fn(o, psi_ospre, __VA_ARGS__); break;                            \
+    case 6: fn(o, psi_vanalbada1, __VA_ARGS__); break;                       \
+    case 7: fn(o, psi_umist, __VA_ARGS__); break;                            \
+    case 8: fn(o, psi_upwind, __VA_ARGS__); break;                           \
+    case 9: fn(o, psi_laxwendroff, __VA_ARGS__); break;                      \
+    }
+
+/* ---------------- hyperviscosity Laplacians (laplacian.c:51-151) -------- */
+static void laplacian_vels_o(orc *o, double *uin, double *vin, double *uout, double *vout) {
+    int nx = o->nx, ny = o->ny;
+    double *ze = o->ze, *di = o->di;
+    memset(ze, 0, sizeof(double) * o->S);
+    memset(di, 0, sizeof(double) * o->S);
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            double a = (1.0 / o->dxv[RJ(j)]), b = -(1.0 / o->dyc[RJ(j + 1)]);
+            F(ze, i, j) = a * ((F(vin, i + 1, j) - F(vin, i, j)) + b * (o->dxc[RJ(j + 1)] * F(uin, i, j + 1) - o->dxc[RJ(j)] * F(uin, i, j)));
+            a = (1.0 / o->dxe[RJ(j)]);
+            b = (1.0 / o->dyn[RJ(j)]);
+            F(di, i, j) = a * ((F(uin, i, j) - F(uin, i - 1, j)) + b * (o->dxv[RJ(j)] * F(vin, i, j) - o->dxv[RJ(j - 1)] * F(vin, i, j - 1)));
+        }
+    halo_update_o(o, ze);
+    halo_update_o(o, di);
+    for (int i = 1; i <= nx; i++) F(di, i, ny + 1) = (F(uin, i, ny) - F(uin, i - 1, ny)) / o->dxe[RJ(ny)];
+    for (int i = 1; i <= nx; i++) F(ze, i, 0) = 0;
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            double a = -(1.0 / o->dyn[RJ(j)]), b = (1.0 / o->dxc[RJ(j)]);
+            F(uout, i, j) = a * (F(ze, i, j) - F(ze, i, j - 1)) + b * (F(di, i + 1, j) - F(di, i, j));
+            a = (1.0 / o->dxv[RJ(j)]);
+            b = (1.0 / o->dyc[RJ(j + 1)]);
+            F(vout, i, j) = a * (F(ze, i, j) - F(ze, i - 1, j)) + b * (F(di, i, j + 1) - F(di, i, j));
+        }
+    halo_update_o(o, uout);
+    halo_update_o(o, vout);
+    for (int i = 1; i <= nx; i++) {
+        F(uout, i, ny + 1) = F(uout, i, ny);
+        F(vout, i, ny + 1) = 0;
+    }
+    for (int i = 1; i <= nx; i++) {
+        F(uout, i, 0) = F(uout, i, 1);
+        F(vout, i, 0) = 0;
+    }
+}
+
+static void laplacian_eta_o(orc *o, double *ein, double *eout) {
+    int nx = o->nx, ny = o->ny;
+    /* psi and xi are fresh allocations in SW2; their halo entries that the
+     * second loop reads (psi(0,j), xi(i,0)) are never written: zero here */
+    memset(o->psi, 0, sizeof(double) * o->S);
+    memset(o->xi, 0, sizeof(double) * o->S);
+    for (int j = 1; j <= ny; j++) {
+        double latc = LATC(j), latn = LATN(j);
+        double fpsi = RM(latc) / RZ(latc), fxi = RZ(latn) / RM(latn);
+        for (int i = 1; i <= nx; i++) {
+            F(o->psi, i, j) = fpsi * (F(ein, i + 1, j) - F(ein, i, j)) / (o->dxc[RJ(j)]);
+            F(o->xi, i, j) = fxi * (F(ein, i, j + 1) - F(ein, i, j)) / (o->dyc[RJ(j)]);
+        }
+    }
+    for (int j = 1; j <= ny; j++) {
+        double latc = LATC(j);
+        double k = 1.0 / (RM(latc) * RZ(latc));
+        for (int i = 1; i <= nx; i++)
+            F(eout, i, j) = k * ((F(o->psi, i, j) - F(o->psi, i - 1, j)) / (o->dxe[RJ(j)]) + (F(o->xi, i, j) - F(o->xi, i, j - 1)) / (o->dyn[RJ(j)]));
+    }
+    halo_update_o(o, eout);
+    for (int i = 1; i <= nx; i++) {
+        F(eout, i, ny + 1) = F(eout, i, ny);
+        F(eout, i, ny + 2) = F(eout, i, ny + 1);
+    }
+    for (int i = 1; i <= nx; i++) F(eout, i, 0) = F(eout, i, 1); /* (i,-1) is left as is */
+}
+
+/* ---------------- AB, pressure, Coriolis, CFL, stats -------------------- */
+static void adams_bashforth(orc *o, double *nab, double *d, double *m1, double *m2, int k) {
+    for (int j = 1; j <= o->ny; j++)
+        for (int i = 1; i <= o->nx; i++) {
+            if (k == 0)
+                F(nab, i, j) = F(d, i, j); /* copy_scaf, calc.c:683 */
+            else if (k == 1)
+                F(nab, i, j) = (3.0 / 2.0) * F(d, i, j) + (-(1.0 / 2.0)) * F(m1, i, j); /* linop_scaf, calc.c:686 */
+            else {
+                double t = (-(4.0 / 3.0)) * F(m1, i, j) + (+(5.0 / 12.0)) * F(m2, i, j); /* calc.c:690 */
+                F(nab, i, j) = (23.0 / 12.0) * F(d, i, j) + 1 * t;                       /* calc.c:692 */
+            }
+        }
+}
+
+static double compute_dt_o(orc *o) {
+    double max_x = 0, max_y = 0;
+    for (int j = 1; j <= o->ny; j++)
+        for (int i = 1; i <= o->nx; i++) {
+            if (fabs(UW(i, j) / o->dxe[RJ(j)]) > max_x) max_x = fabs(UW(i, j)) / o->dxe[RJ(j)];
+            if (fabs(VW(i, j) / o->dyn[RJ(j)]) > max_y) max_y = fabs(VW(i, j)) / o->dyn[RJ(j)];
+        }
+    if (max_x + max_y == 0) {
+        for (int j = 1; j <= o->ny; j++)
+            for (int i = 1; i <= o->nx; i++) {
+                if (fabs(sqrt((o->depth) * (o->ge[RJ(j)])) / o->dxe[RJ(j)]) > max_x) max_x = fabs(sqrt((o->depth) * (o->ge[RJ(j)]))) / o->dxe[RJ(j)];
+                if (fabs(sqrt((o->depth) * (o->gn[RJ(j)])) / o->dyn[RJ(j)]) > max_y) max_y = fabs(sqrt((o->depth) * (o->gn[RJ(j)]))) / o->dyn[RJ(j)];
+            }
+    }
+    return (o->Courant / (max_x + max_y));
+}
+
+/* get_stats_scaf + check_stats_scaf (sppde_core.c:346-395) */
+static int check_stats_o(orc *o, double *r, const char *name, double *smin, double *smax, double *savg) {
+    double s = 0, n = 0, mn = 0, mx = 0;
+    for (int j = 1; j <= o->ny; j++)
+        for (int i = 1; i <= o->nx; i++) {
+            n = n + 1;
+            s += F(r, i, j);
+            if (F(r, i, j) < mn || n == 1.0) mn = F(r, i, j);
+            if (F(r, i, j) > mx || n == 1.0) mx = F(r, i, j);
+        }
+    if (smin) *smin = mn;
+    if (smax) *smax = mx;
+    if (savg) *savg = s / n;
+    if (mn < -1e10 || mx > 1e10 || isfinite(mn) == 0 || isfinite(mx) == 0) {
+        snprintf(o->failmsg, sizeof o->failmsg, "OOOPS nonsense value detected in %s min=%e max=%e", name, mn, mx);
+        return 1;
+    }
+    return 0;
+}
+
+/* ---------------- storm injection and vortices (perturbs.c:22-460) ------ */
+static double great_circle(double lon0, double lat0, double lon1, double lat1, double r) {
+    return (r) * acos(sin(lat0) * sin(lat1) + cos(lat0) * cos(lat1) * cos(lon1 - lon0)); /* tools.c:104-119 */
+}
+
+static void gaussian_field(orc *o, double lon, double lat, double amp, double sigma, double radius, double *field, double inject) {
+    for (int j = 1; j <= o->ny; j++)
+        for (int i = 1; i <= o->nx; i++) {
+            double r = great_circle(lon, L2PC(lat), LONC(i), L2PC(LATC(j)), (o->rE + o->rP) / 2.0);
+            if (r < radius) {
+                F(field, i, j) += amp * exp(-(r * r) / (2.0 * (sigma * sigma)));
+                F(o->tracer, i, j) = (inject) * exp(-(r * r) / (2.0 * (sigma * sigma)));
+                F(o->perturbs, i, j) = 1.0;
+            }
+        }
+}
+
+static int gaussian_core(orc *o, double lon, double lat, double volume, double sigma, double radius, double inject) {
+    while (lon < o->lon0) lon += (o->lon1 - o->lon0);
+    while (lon > o->lon1) lon -= (o->lon1 - o->lon0);
+    double amp = volume / (2.0 * OPI * (sigma * sigma) * (1.0 - exp(-(radius * radius) / (2.0 * (sigma * sigma)))));
+    double *pf = o->pf;
+    memset(pf, 0, sizeof(double) * o->S);
+    gaussian_field(o, lon, lat, amp, sigma, radius, pf, inject);
+    double lonRight = lon + (o->lon1 - o->lon0);
+    double lonLeft = lon - (o->lon1 - o->lon0);
+    if ((lon + radius / RZ(lat)) >= o->lon1) gaussian_field(o, lonLeft, lat, amp, sigma, radius, pf, inject);
+    if ((lon - radius / RZ(lat)) <= o->lon0) gaussian_field(o, lonRight, lat, amp, sigma, radius, pf, inject);
+    double V = 0.0;
+    for (int j = 1; j <= o->ny; j++)
+        for (int i = 1; i <= o->nx; i++) {
+            double a = F(pf, i, j);
+            double s = o->dxc[RJ(j)] * o->dyc[RJ(j)];
+            double v = a * s;
+            V += v;
+        }
+    double Vg = V;
+    double Vcheck = 0.0;
+    if (fabs(volume) > 1e-7) {
+        if (Vg == 0.0) {
+            snprintf(o->failmsg, sizeof o->failmsg, "Global total injected volume Vg=%e equals 0", Vg);
+            return 1;
+        }
+        double rf = (volume / Vg);
+        for (int j = 1; j <= o->ny; j++)
+            for (int i = 1; i <= o->nx; i++) {
+                double a = rf * F(pf, i, j);
+                F(o->eta, i, j) += a;
+                double s = o->dxc[RJ(j)] * o->dyc[RJ(j)];
+                double v = a * s;
+                Vcheck += v;
+            }
+        if (fabs(Vcheck - volume) / volume > 1e-1) {
+            snprintf(o->failmsg, sizeof o->failmsg, "Total injected volume Vcheck=%e does not match input volume=%e", Vcheck, volume);
+            return 1;
+        }
+    }
+    return 0;
+}
+
+static int add_gaussians_o(orc *o, double Dt, double t) {
+    for (int j = 0; j <= o->nGauss - 1; j++) {
+        double *g = o->Gauss + j * N_COLS_GAUSS;
+        double t0g = g[0], t1g = g[1];
+        if ((t0g <= t) && (t1g >= t)) {
+            double tau = g[2];
+            if (tau < 0) { snprintf(o->failmsg, sizeof o->failmsg, "tau is negative"); return 1; }
+            double lon_g = DEG2RAD(g[3]);
+            double lat_g = DEG2RAD(g[4]);
+            int isdegvel = (int)g[5];
+            double vel = g[6];
+            if (isdegvel < 0 || isdegvel > 1) { snprintf(o->failmsg, sizeof o->failmsg, "isdegvel should be 0 or 1"); return 1; }
+            double tt = (t - t0g);
+            double Om = 0;
+            if (isdegvel == 0) Om = vel / RZ(lat_g);
+            if (isdegvel == 1) Om = DEG2RAD(vel);
+            double lon_g0 = lon_g;
+            lon_g = lon_g0 + Om * tt;
+            double Q = g[7];
+            int isdegsize = (int)g[8];
+            double sigma = g[9], radius = g[10], inject = g[11];
+            if (isdegsize < 0 || isdegsize > 1) { snprintf(o->failmsg, sizeof o->failmsg, "isdegsize should be 0 or 1"); return 1; }
+            if (isdegsize == 1) {
+                sigma = sigma / 360.0 * 2 * OPI * (o->rE + o->rP) / 2.0;
+                radius = radius / 360.0 * 2 * OPI * (o->rE + o->rP) / 2.0;
+            }
+            double Qexp = 0; /* SW2 leaves it unset when no branch applies */
+            double tauTH = 6 * tau;
+            double t_int = tau * log(exp((t1g - tauTH) / tau) + exp(t0g / tau));
+            if ((t < tauTH + t0g) && (t < t_int)) Qexp = Q * (1.0 - exp(-(t - t0g) / tau));
+            if ((t0g + tauTH <= t) && (t <= t1g - tauTH)) Qexp = Q;
+            if ((t > t1g - tauTH) && (t > t_int)) Qexp = Q * exp(-(t - (t1g - tauTH)) / tau);
+            if (gaussian_core(o, lon_g, lat_g, Dt * Qexp, sigma, radius, inject)) return 1;
+        }
+    }
+    return 0;
+}
+
+static void add_vortices_o(orc *o, double Dt, double t) {
+    for (int k = 0; k <= o->nVort - 1; k++) {
+        double *w = o->Vort + k * N_COLS_VORTICES;
+        double t0v = w[0];
+        if ((t0v < t) && (t0v >= (t - Dt))) {
+            double lonc = w[1], latc = w[2], as = w[3], bs = w[4], Umax = w[5], n = w[6];
+            double amp = 0.5 * Umax * RM(DEG2RAD(latc)) * DEG2RAD(bs) * FCOR(DEG2RAD(latc)) *
+                         pow((1.0 - 1.0 / (2.0 * n)), ((1.0 - 2.0 * n) / (2.0 * n))) * exp(1.0 - 1.0 / (2.0 * n)) /
+                         (GEFF(DEG2RAD(latc)) * n);
+            for (int j = 1; j <= o->ny; j++)
+                for (int i = 1; i <= o->nx; i++) {
+                    double ex = pow(((RAD2DEG(LONC(i)) - lonc) / as) * ((RAD2DEG(LONC(i)) - lonc) / as) +
+                                        ((RAD2DEG(LATC(j)) - latc) / bs) * ((RAD2DEG(LATC(j)) - latc) / bs),
+                                    n);
+                    double eij = amp * exp(-ex);
+                    F(o->eta, i, j) += eij;
+                    double pu = ((RAD2DEG(LONE(i)) - lonc) / as) * ((RAD2DEG(LONE(i)) - lonc) / as) +
+                                ((RAD2DEG(LATC(j)) - latc) / bs) * ((RAD2DEG(LATC(j)) - latc) / bs);
+                    F(o->u, i, j) += 2.0 * (n)*o->ge[RJ(j)] / (FCOR(DEG2RAD(latc)) * (DEG2RAD(bs) * DEG2RAD(bs)) * RM(DEG2RAD(latc))) *
+                                     (LATC(j) - DEG2RAD(latc)) * pow(pu, n - 1.0) * eij;
+                    double pv = ((RAD2DEG(LONC(i)) - lonc) / as) * ((RAD2DEG(LONC(i)) - lonc) / as) +
+                                ((RAD2DEG(LATN(j)) - latc) / bs) * ((RAD2DEG(LATN(j)) - latc) / bs);
+                    F(o->v, i, j) += -2.0 * (n)*o->gn[RJ(j)] / (FCOR(DEG2RAD(latc)) * (DEG2RAD(as) * DEG2RAD(as)) * RZ(DEG2RAD(latc))) *
+                                     (LONC(i) - DEG2RAD(lonc)) * pow(pv, n - 1.0) * eij;
+                }
+            halo_update_o(o, o->eta);
+            halo_update_o(o, o->u);
+            halo_update_o(o, o->v);
+        }
+    }
+}
+
+/* ---------------- wind table (io.c:173-244, tools.c:21-60) ------------- */
+static int linterp_o(double xi, const double *x, const double *y, int n, double *out) {
+    int i = 0, i1 = n - 1;
+    if (x[0] > xi || x[i1] < xi) return 1;
+    while ((x[i + 1] <= xi) && (i < i1) && (i >= 0)) i++;
+    *out = (y[i + 1] - y[i]) * (xi - x[i]) / (x[i + 1] - x[i]) + y[i];
+    return 0;
+}
+
+/* ---------------- public API ------------------------------------------- */
+orc *orc_create(const sw2_params *P) {
+    if (P->polar != 0) return NULL;
+    orc *o = (orc *)calloc(1, sizeof(orc));
+    o->nx = P->nx;
+    o->ny = P->ny;
+    o->sx = P->nx + 4;
+    o->S = (P->nx + 4) * (P->ny + 4);
+    o->sch = P->sch;
+    /* init.c:56-59 */
+    o->lon0 = DEG2RAD(P->lon0_deg);
+    o->lon1 = DEG2RAD(P->lon1_deg);
+    o->lat0 = DEG2RAD(P->lat0_deg);
+    o->lat1 = DEG2RAD(P->lat1_deg);
+    o->Dt = P->Dt;
+    o->Courant = P->CFL;
+    o->N = (int)(((P->t1) - (P->t0)) / (P->Dt)); /* init.c:109 */
+    o->sigma_dis = (P->Tau > 1e10) ? 0 : 1.0 / P->Tau; /* init.c:110-114 */
+    o->nu2 = P->hypernu[0];
+    o->nu4 = -P->hypernu[1];
+    o->nu6 = P->hypernu[2];
+    o->IteInfo = P->IteInfo;
+    o->Omega = P->Omega;
+    o->rE = P->rE;
+    o->rP = P->rP;
+    o->eps2 = ((o->rE) / (o->rP)) * ((o->rE) / (o->rP)); /* init.c:146 */
+    o->gE = P->gE;
+    o->depth = P->depth;
+    o->tracer_dis = P->TracerDis;
+    o->t = P->t0;
+    int S = o->S, R = P->ny + 4;
+    double **fields[] = {&o->eta, &o->u, &o->v, &o->tracer, &o->perturbs, &o->hB, &o->windx, &o->windy, &o->Trab, &o->Etab,
+                         &o->Duab, &o->Dvab, &o->Duadv, &o->Dvadv, &o->Dupres, &o->Dvpres, &o->Lu, &o->Lv, &o->LLu, &o->LLv,
+                         &o->LLLu, &o->LLLv, &o->ze, &o->di, &o->le, &o->lle, &o->llle, &o->psi, &o->xi, &o->pf};
+    for (size_t k = 0; k < sizeof fields / sizeof fields[0]; k++) *fields[k] = zalloc(S);
+    for (int k = 0; k < 3; k++) {
+        o->Dtr[k] = zalloc(S);
+        o->Deta[k] = zalloc(S);
+        o->Du[k] = zalloc(S);
+        o->Dv[k] = zalloc(S);
+    }
+    double **rows[] = {&o->dxe, &o->dxc, &o->dxv, &o->dyn, &o->dyc, &o->dyv, &o->ge, &o->gn, &o->gc, &o->gv, &o->fe, &o->fn, &o->fc, &o->fv};
+    for (size_t k = 0; k < sizeof rows / sizeof rows[0]; k++) *rows[k] = zalloc(R);
+    geometry(o);
+    o->nGauss = P->num_Gaussians;
+    o->Gauss = zalloc(o->nGauss * N_COLS_GAUSS + 1);
+    if (o->nGauss) memcpy(o->Gauss, P->Gaussians, sizeof(double) * o->nGauss * N_COLS_GAUSS);
+    o->nVort = P->num_vortices;
+    o->Vort = zalloc(o->nVort * N_COLS_VORTICES + 1);
+    if (o->nVort) memcpy(o->Vort, P->vortices, sizeof(double) * o->nVort * N_COLS_VORTICES);
+    /* wind: io.c:199-205 then halo updates of init.c:232-235 */
+    if (P->wind_rows > 0) {
+        int nr = P->wind_rows;
+        double *x = zalloc(nr), *y = zalloc(nr);
+        for (int r = 0; r < nr; r++) {
+            x[r] = P->wind[2 * r];
+            y[r] = P->wind[2 * r + 1];
+        }
+        for (int j = -1; j <= o->ny + 2; j++)
+            for (int i = -1; i <= o->nx + 2; i++) {
+                double w = 0;
+                if (linterp_o(RAD2DEG(LATC(j)), x, y, nr, &w)) {
+                    free(x);
+                    free(y);
+                    return NULL;
+                }
+                F(o->windx, i, j) = w;
+            }
+        free(x);
+        free(y);
+    }
+    halo_update_o(o, o->windx);
+    halo_update_o(o, o->windy);
+    return o;
+}
+
+void orc_free(orc *o) {
+    if (!o) return;
+    double *fields[] = {o->eta, o->u, o->v, o->tracer, o->perturbs, o->hB, o->windx, o->windy, o->Trab, o->Etab, o->Duab, o->Dvab,
+                        o->Duadv, o->Dvadv, o->Dupres, o->Dvpres, o->Lu, o->Lv, o->LLu, o->LLv, o->LLLu, o->LLLv, o->ze, o->di,
+                        o->le, o->lle, o->llle, o->psi, o->xi, o->pf, o->dxe, o->dxc, o->dxv, o->dyn, o->dyc, o->dyv, o->ge,
+                        o->gn, o->gc, o->gv, o->fe, o->fn, o->fc, o->fv, o->Gauss, o->Vort};
+    for (size_t k = 0; k < sizeof fields / sizeof fields[0]; k++) free(fields[k]);
+    for (int k = 0; k < 3; k++) {
+        free(o->Dtr[k]);
+        free(o->Deta[k]);
+        free(o->Du[k]);
+        free(o->Dv[k]);
+    }
+    free(o);
+}
+
+/* which: 0 eta 1 u 2 v 3 tracer 4 perturbs 5 windx 6 windy 7 hB */
+double *orc_field(orc *o, int which) {
+    double *f[] = {o->eta, o->u, o->v, o->tracer, o->perturbs, o->windx, o->windy, o->hB};
+    return (which >= 0 && which < 8) ? f[which] : NULL;
+}
+/* which: 0 dxe 1 dxc 2 dxv 3 dyn 4 dyc 5 dyv 6 ge 7 gn 8 gc 9 gv 10 fe 11 fn 12 fc 13 fv */
+double *orc_rows(orc *o, int which) {
+    double *f[] = {o->dxe, o->dxc, o->dxv, o->dyn, o->dyc, o->dyv, o->ge, o->gn, o->gc, o->gv, o->fe, o->fn, o->fc, o->fv};
+    return (which >= 0 && which < 14) ? f[which] : NULL;
+}
+int orc_size(orc *o) { return o->S; }
+int orc_nsteps(orc *o) { return o->N; }
+double orc_time(orc *o) { return o->t; }
+int orc_step_index(orc *o) { return o->n; }
+const char *orc_failmsg(orc *o) { return o->failmsg; }
+double orc_last_dtcfl(orc *o) { return o->last_dtcfl; }
+
+/* Pre-loop of loop() (core.c:87-101).  Returns 0 or 1 on a fatal error. */
+int orc_preloop(orc *o) {
+    halo_update_o(o, o->tracer);
+    halo_update_o(o, o->u);
+    halo_update_o(o, o->v);
+    halo_update_o(o, o->eta);
+    halo_update_o(o, o->hB);
+    memset(o->perturbs, 0, sizeof(double) * o->S);
+    if (add_gaussians_o(o, o->Dt, o->t)) return 1;
+    add_vortices_o(o, o->Dt, o->t);
+    halo_update_o(o, o->tracer);
+    halo_update_o(o, o->eta);
+    do_channel_o(o);
+    return 0;
+}
+
+/* What load_binary_sw leaves after the caller wrote the interiors of eta, u,
+ * v, tracer and perturbs (sppde_binary_files.c:147-180: scatter interior,
+ * then halo_update). */
+void orc_after_load(orc *o, int nstep, double t) {
+    halo_update_o(o, o->eta);
+    halo_update_o(o, o->u);
+    halo_update_o(o, o->v);
+    halo_update_o(o, o->tracer);
+    halo_update_o(o, o->perturbs);
+    o->nstart = nstep;
+    o->n = nstep;
+    o->t = t;
+}
+
+void orc_begin(orc *o, int nstart, double t) {
+    o->nstart = nstart;
+    o->n = nstart;
+    o->t = t;
+}
+
+/* KE and APE exactly as calc_energy_balance (energy.c:25-55) */
+void orc_energy(orc *o, double *ke, double *ape) {
+    double myke = 0, myape = 0, uc, vc, mysurf;
+    for (int j = 1; j <= o->ny; j++)
+        for (int i = 1; i <= o->nx; i++) {
+            uc = 0.5 * (UW(i, j) + UW(i - 1, j));
+            vc = 0.5 * (VW(i, j) + VW(i, j - 1));
+            mysurf = o->dxc[RJ(j)] * o->dyc[RJ(j)];
+            myke += 0.5 * HH(i, j) * (uc * uc + vc * vc) * mysurf - 0.5 * ((F(o->windx, i, j) * F(o->windx, i, j)) + (F(o->windy, i, j) * F(o->windy, i, j))) * mysurf;
+            myape += 0.5 * o->gc[RJ(j)] * (HH(i, j) * HH(i, j)) * mysurf - 0.5 * o->gc[RJ(j)] * ((o->depth) * (o->depth)) * mysurf;
+        }
+    *ke = myke;
+    *ape = myape;
+}
+
+/* sum of H*Dxc*Dyc over the interior, SW2's loop order */
+double orc_mass(orc *o) {
+    double m = 0;
+    for (int j = 1; j <= o->ny; j++)
+        for (int i = 1; i <= o->nx; i++) m += HH(i, j) * (o->dxc[RJ(j)] * o->dyc[RJ(j)]);
+    return m;
+}
+
+static void rotate(double *h[3]) {
+    /* copy_scaf(nM2,nM1); copy_scaf(nM1,n): the interior values move down a
+     * slot; SW2 copies interiors only, and no halo of a history array is read */
+    double *m2 = h[2];
+    h[2] = h[1];
+    h[1] = h[0];
+    h[0] = m2;
+}
+
+/* One step of loop() (core.c:134-338) without IO.  Returns 0, or 1 when SW2
+ * would CRASH at this step (message in orc_failmsg). */
+static int step_once(orc *o) {
+    int n = o->n + 1;
+    int k = n - o->nstart - 1; /* n - nstart with nstart already incremented */
+    double Dt = o->Dt;
+    int nx = o->nx, ny = o->ny;
+    o->prn = (k == 0 || n % o->IteInfo == 0);
+    o->t += Dt;
+    /* compute_Dt (calc.c:701-738) and the guard of core.c:158 */
+    double dtcfl = compute_dt_o(o);
+    o->last_dtcfl = dtcfl;
+    if (Dt > dtcfl) {
+        snprintf(o->failmsg, sizeof o->failmsg, "Smallest possible timestep is Dt_CFL=%.6e while Dt=%.6e.", dtcfl, Dt);
+        o->n = n;
+        return 1;
+    }
+    /* tracer (core.c:161-182) */
+    DISPATCH(body_tracer, o->Dtr[0], Dt)
+    adams_bashforth(o, o->Trab, o->Dtr[0], o->Dtr[1], o->Dtr[2], k);
+    rotate(o->Dtr);
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) F(o->tracer, i, j) = F(o->tracer, i, j) + F(o->Trab, i, j);
+    halo_update_o(o, o->tracer);
+    /* eta (core.c:184-210; calc.c:289-355, 403-422) */
+    DISPATCH(body_h, o->Deta[0], Dt)
+    if (o->nu2 != 0 || o->nu4 != 0 || o->nu6 != 0) {
+        memset(o->le, 0, sizeof(double) * o->S);
+        memset(o->lle, 0, sizeof(double) * o->S);
+        memset(o->llle, 0, sizeof(double) * o->S);
+        laplacian_eta_o(o, o->u, o->le); /* SW2 passes u here (calc.c:332) */
+        if (o->nu4 != 0 || o->nu6 != 0) {
+            laplacian_eta_o(o, o->le, o->lle);
+            if (o->nu6 != 0) laplacian_eta_o(o, o->lle, o->llle);
+        }
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                F(o->Deta[0], i, j) += (o->nu2) * (o->Dt) * F(o->le, i, j);
+                F(o->Deta[0], i, j) += (o->nu4) * (o->Dt) * F(o->lle, i, j);
+                F(o->Deta[0], i, j) += (o->nu6) * (o->Dt) * F(o->llle, i, j);
+            }
+    }
+    adams_bashforth(o, o->Etab, o->Deta[0], o->Deta[1], o->Deta[2], k);
+    for (int j = 1; j <= ny; j++) {
+        double gf = Dt * sin(LATC(j)) / RZ(LATC(j));
+        for (int i = 1; i <= nx; i++) F(o->Etab, i, j) -= (gf * HH(i, j) * 0.5 * (VW(i, j) + VW(i, j - 1)));
+    }
+    rotate(o->Deta);
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) F(o->eta, i, j) = F(o->eta, i, j) + F(o->Etab, i, j);
+    halo_update_o(o, o->eta);
+    /* velocities (core.c:212-280) */
+    DISPATCH(body_u, o->Duadv, Dt)
+    DISPATCH(body_v, o->Dvadv, Dt)
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            F(o->Dupres, i, j) = -o->ge[RJ(j)] * Dt * (F(o->eta, i + 1, j) - F(o->eta, i, j)) / o->dxc[RJ(j)];
+            F(o->Dvpres, i, j) = -o->gn[RJ(j)] * Dt * (F(o->eta, i, j + 1) - F(o->eta, i, j)) / o->dyc[RJ(j + 1)];
+        }
+    double *Dun = o->Du[0], *Dvn = o->Dv[0];
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            F(Dun, i, j) = F(o->Duadv, i, j) + F(o->Dupres, i, j);
+            F(Dvn, i, j) = F(o->Dvadv, i, j) + F(o->Dvpres, i, j);
+        }
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) { /* dissipation_effect, calc.c:396-397 */
+            F(Dun, i, j) -= F(o->u, i, j) * o->sigma_dis * o->Dt;
+            F(Dvn, i, j) -= F(o->v, i, j) * o->sigma_dis * o->Dt;
+        }
+    if (o->nu2 != 0 || o->nu4 != 0 || o->nu6 != 0) {
+        double *zs[] = {o->Lu, o->Lv, o->LLu, o->LLv, o->LLLu, o->LLLv};
+        for (int q = 0; q < 6; q++) memset(zs[q], 0, sizeof(double) * o->S);
+        laplacian_vels_o(o, o->u, o->v, o->Lu, o->Lv);
+        if (o->nu4 != 0 || o->nu6 != 0) {
+            laplacian_vels_o(o, o->Lu, o->Lv, o->LLu, o->LLv);
+            if (o->nu6 != 0) laplacian_vels_o(o, o->LLu, o->LLv, o->LLLu, o->LLLv);
+        }
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                F(Dun, i, j) += (o->nu2) * (o->Dt) * F(o->Lu, i, j);
+                F(Dun, i, j) += (o->nu4) * (o->Dt) * F(o->LLu, i, j);
+                F(Dun, i, j) += (o->nu6) * (o->Dt) * F(o->LLLu, i, j);
+                F(Dvn, i, j) += (o->nu2) * (o->Dt) * F(o->Lv, i, j);
+                F(Dvn, i, j) += (o->nu4) * (o->Dt) * F(o->LLv, i, j);
+                F(Dvn, i, j) += (o->nu6) * (o->Dt) * F(o->LLLv, i, j);
+            }
+    }
+    adams_bashforth(o, o->Duab, Dun, o->Du[1], o->Du[2], k);
+    adams_bashforth(o, o->Dvab, Dvn, o->Dv[1], o->Dv[2], k);
+    rotate(o->Du);
+    rotate(o->Dv);
+    /* Coriolis, u then v, in place (calc.c:542-598) */
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            double semiU = 0.25 * (F(o->v, i, j) + F(o->v, i + 1, j) + F(o->v, i, j - 1) + F(o->v, i + 1, j - 1));
+            double Alpha = Dt * o->fe[RJ(j)] + (Dt * F(o->u, i, j) * sin(LATC(j))) / RZ(LATC(j));
+            double Beta = 0.25 * Alpha * Alpha;
+            F(o->u, i, j) = (F(o->u, i, j) + F(o->Duab, i, j) - Beta * F(o->u, i, j) + Alpha * semiU) / (1.0 + Beta);
+        }
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            double semiV = 0.25 * (F(o->u, i - 1, j + 1) + F(o->u, i, j + 1) + F(o->u, i, j) + F(o->u, i - 1, j));
+            double Alpha = Dt * o->fn[RJ(j)] + (Dt * semiV * sin(LATN(j))) / RZ(LATN(j));
+            double Beta = 0.25 * Alpha * Alpha;
+            F(o->v, i, j) = (F(o->v, i, j) + F(o->Dvab, i, j) - Beta * F(o->v, i, j) - Alpha * semiV) / (1.0 + Beta);
+        }
+    halo_update_o(o, o->u);
+    halo_update_o(o, o->v);
+    /* perturbations (core.c:292-300) */
+    memset(o->perturbs, 0, sizeof(double) * o->S);
+    o->n = n;
+    if (add_gaussians_o(o, Dt, o->t)) return 1;
+    add_vortices_o(o, Dt, o->t);
+    halo_update_o(o, o->tracer);
+    halo_update_o(o, o->eta);
+    do_channel_o(o);
+    if (check_stats_o(o, o->tracer, "tracer", 0, 0, 0)) return 1;
+    if (check_stats_o(o, o->eta, "eta", 0, 0, 0)) return 1;
+    if (check_stats_o(o, o->u, "u", 0, 0, 0)) return 1;
+    if (check_stats_o(o, o->v, "v", 0, 0, 0)) return 1;
+    return 0;
+}
+
+/* Run nsteps steps.  Returns 0, or the step at which SW2 would CRASH.
+ * If rec is given, (t, KE, APE) is stored at every step SW2 prints ENERGY
+ * (core.c:136, 328-331) and *nrec counts the records. */
+int orc_step(orc *o, int nsteps, double *rec, int *nrec, int maxrec) {
+    for (int s = 0; s < nsteps; s++) {
+        if (o->failed_step) return o->failed_step;
+        if (step_once(o)) {
+            o->failed_step = o->n + (o->n == 0);
+            return o->failed_step;
+        }
+        if (rec && o->prn && *nrec < maxrec) {
+            double ke, ape;
+            orc_energy(o, &ke, &ape);
+            rec[3 * *nrec] = o->t;
+            rec[3 * *nrec + 1] = ke;
+            rec[3 * *nrec + 2] = ape;
+            (*nrec)++;
+        }
+    }
+    return 0;
+}
+
+/* check_stats values of one field (min, max, avg), for tests of the guard */
+int orc_stats(orc *o, int which, double *out3) {
+    double *f = orc_field(o, which);
+    if (!f) return -1;
+    return check_stats_o(o, f, "field", &out3[0], &out3[1], &out3[2]);
+}
+
+/* Test hooks: SW2's two Laplacians applied to caller fields (interiors
+ * given, halos set here by the same halo rules as in the step). */
+void orc_apply_laplacian_vels(orc *o, double *uin, double *vin, double *uout, double *vout) {
+    halo_update_o(o, uin);
+    halo_update_o(o, vin);
+    laplacian_vels_o(o, uin, vin, uout, vout);
+}
+
+void orc_apply_laplacian_eta(orc *o, double *ein, double *eout) {
+    halo_update_o(o, ein);
+    laplacian_eta_o(o, ein, eout);
+}

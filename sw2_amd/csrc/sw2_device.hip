@@ -1,0 +1,897 @@
+// sw2_device.hip -- GPU time step of SW2 for gfx950 (MI355X), FP64 form.
+//
+// One device session holds a world's fields in HBM with SW2's sppde layout
+// (halo of 2, x fastest) and advances it with the per-step order of loop()
+// (core.c:134-338).  Each phase is a kernel over the interior with one lane
+// per cell, longitude-major so that a wavefront reads 64 consecutive cells of
+// a row.  Halo updates reproduce the single-map rules of
+// sppde/sppde_hupdate.c:274-363 (periodic images in x and in y, corners
+// included) and do_channel (calc.c:600-649) rewrites interior columns only,
+// so the halo state -- and with it every value -- is the one SW2's CPU path
+// computes on one rank.
+//
+// Arithmetic: FP64, compiled with -ffp-contract=off so no multiply-add is
+// fused; every expression keeps SW2's association; divisions are IEEE
+// correctly rounded; transcendental row values come from the host libm.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "sw2_internal.h"
+
+#define HIPCHECK(x)                                                                     \
+    do {                                                                                \
+        hipError_t e_ = (x);                                                            \
+        if (e_ != hipSuccess) {                                                         \
+            sw2_set_error("HIP error %s at %s:%d", hipGetErrorString(e_), __FILE__, __LINE__); \
+            return -1;                                                                  \
+        }                                                                               \
+    } while (0)
+
+namespace {
+
+constexpr int BX = 64;  // lanes along a row: one wavefront per row segment
+constexpr int BY = 4;   // rows per workgroup
+
+struct Grid {
+    int nx, ny, sx;     // interior size, row stride (nx+4)
+    int R;              // row-table length (ny+4)
+    const double *T;    // row tables, SW2_NT x R
+};
+
+__device__ __forceinline__ int IX(const Grid &g, int i, int j) { return (i + 1) + (j + 1) * g.sx; }
+__device__ __forceinline__ double TB(const Grid &g, int t, int j) { return g.T[t * g.R + (j + 1)]; }
+
+// fail word: step of the first failed guard (0 = none), kind, value
+struct Flags {
+    unsigned long long cfl[4];  // two slots of (max_x, max_y) bit patterns
+    int fail_step;
+    int fail_kind;              // 1 CFL, 2 tracer, 3 eta, 4 u, 5 v
+    // set once u or v holds a non-finite value: from then on an identically
+    // zero tracer no longer stays zero in SW2 (0*NaN), so its kernels run
+    int nonfinite;
+    double fail_val;
+    double last_dtcfl;
+};
+
+// ------------------------------------------------------------ limiters
+// calc.c:54-63; MIN/MAX are the ternaries of sw.h:104-105
+#define DMIN(a, b) (((a) < (b)) ? (a) : (b))
+#define DMAX(a, b) (((a) > (b)) ? (a) : (b))
+template <int SCH>
+__device__ __forceinline__ double psi(double r) {
+    if constexpr (SCH == 0) return (DMAX(0, DMIN(DMIN((1 + r) / 2.0, 2.0 * r), 2.0)));
+    else if constexpr (SCH == 1) return (DMAX(DMAX(0, DMIN(2.0 * r, 1.0)), DMAX(0, DMIN(r, 2.0))));
+    else if constexpr (SCH == 2) return (DMIN(r, 1));
+    else if constexpr (SCH == 3) return ((r + fabs(r)) / (1.0 + r));
+    else if constexpr (SCH == 4) return (DMAX(0, DMIN(2.0 * r, DMIN((1 + 2.0 * r) / 3.0, 2.0))));
+    else if constexpr (SCH == 5) return ((1.5 * ((r * r) + r)) / ((r * r) + r + 1.0));
+    else if constexpr (SCH == 6) return (((r * r) + r) / ((r * r) + 1.0));
+    else if constexpr (SCH == 7) return (DMAX(0, DMIN(DMIN(2.0, 2.0 * r), DMIN(0.25 + 0.75 * r, 0.75 + 0.25 * r))));
+    else if constexpr (SCH == 8) return (0);
+    else return (1.0);
+}
+
+// calc.c:43-44
+__device__ __forceinline__ double r2rP(double Bprev, double Bcurr, double Bpost) {
+    return (((Bpost - Bcurr) != 0) ? ((Bcurr - Bprev) / (Bpost - Bcurr)) : 0);
+}
+__device__ __forceinline__ double r2rM(double Bcurr, double Bpost, double Bnext) {
+    return (((Bpost - Bcurr) != 0) ? ((Bnext - Bpost) / (Bpost - Bcurr)) : 0);
+}
+// calc.c:48-49
+template <int SCH>
+__device__ __forceinline__ double b2bp(double B1, double B2, double rP, double CP) {
+    return (B1 + psi<SCH>(rP) * 0.5 * (1 - CP) * (B2 - B1));
+}
+template <int SCH>
+__device__ __forceinline__ double b2bm(double B1, double B2, double rM, double CM) {
+    return (B2 - psi<SCH>(rM) * 0.5 * (1 + CM) * (B2 - B1));
+}
+
+// Adams-Bashforth of do_AdamsBashforth (calc.c:680-697) via linop_scaf
+// (sppde_core.c:285-298): Euler, AB2, then AB3
+__device__ __forceinline__ double ab(int k, double d, double m1, double m2) {
+    if (k == 0) return d;
+    if (k == 1) return (3.0 / 2.0) * d + (-(1.0 / 2.0)) * m1;
+    double t = (-(4.0 / 3.0)) * m1 + (+(5.0 / 12.0)) * m2;
+    return (23.0 / 12.0) * d + 1 * t;
+}
+
+struct Ptrs {
+    double *eta, *u, *v, *wx, *wy, *tr;
+};
+
+#define CELL()                                                   \
+    const int i = 1 + blockIdx.x * BX + threadIdx.x;             \
+    const int j = 1 + blockIdx.y * BY + threadIdx.y;             \
+    if (fl->fail_step) return;                                   \
+    if (i > g.nx || j > g.ny) return;
+
+#define UW(i, j) (p.u[IX(g, i, j)] + p.wx[IX(g, i, j)])
+#define VW(i, j) (p.v[IX(g, i, j)] + p.wy[IX(g, i, j)])
+
+// ------------------------------------------------ CFL guard (calc.c:701-738)
+__global__ __launch_bounds__(BX *BY) void k_cfl(Grid g, Ptrs p, Flags *fl, int slot) {
+    const int i = 1 + blockIdx.x * BX + threadIdx.x;
+    const int j = 1 + blockIdx.y * BY + threadIdx.y;
+    if (fl->fail_step) return;
+    double mx = 0, my = 0;
+    if (i <= g.nx && j <= g.ny) {
+        double ax = fabs(UW(i, j)) / TB(g, SW2_T_DXE, j);
+        double ay = fabs(VW(i, j)) / TB(g, SW2_T_DYN, j);
+        if (ax > mx) mx = ax;  // NaN never wins a comparison, as in SW2
+        if (ay > my) my = ay;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        double ox = __shfl_xor(mx, o), oy = __shfl_xor(my, o);
+        mx = ox > mx ? ox : mx;
+        my = oy > my ? oy : my;
+    }
+    __shared__ double sx[BY], sy[BY];
+    if (threadIdx.x == 0) {
+        sx[threadIdx.y] = mx;
+        sy[threadIdx.y] = my;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        for (int q = 1; q < BY; q++) {
+            mx = sx[q] > mx ? sx[q] : mx;
+            my = sy[q] > my ? sy[q] : my;
+        }
+        // non-negative doubles order like their bit patterns
+        atomicMax(&fl->cfl[2 * slot], (unsigned long long)__double_as_longlong(mx));
+        atomicMax(&fl->cfl[2 * slot + 1], (unsigned long long)__double_as_longlong(my));
+    }
+}
+
+__global__ void k_cfl_fin(Flags *fl, int slot, int n, double courant, double Dt, double rest_x, double rest_y) {
+    if (fl->fail_step) return;
+    double mx = __longlong_as_double((long long)fl->cfl[2 * slot]);
+    double my = __longlong_as_double((long long)fl->cfl[2 * slot + 1]);
+    if (mx + my == 0) {  // flow at rest: gravity-wave speeds (calc.c:721-729)
+        mx = rest_x;
+        my = rest_y;
+    }
+    double dtcfl = (courant / (mx + my));
+    fl->last_dtcfl = dtcfl;
+    fl->cfl[2 * (slot ^ 1)] = 0;
+    fl->cfl[2 * (slot ^ 1) + 1] = 0;
+    if (Dt > dtcfl) {
+        fl->fail_step = n;
+        fl->fail_kind = 1;
+        fl->fail_val = dtcfl;
+    }
+}
+
+// --------------------------------------- conservative flux body (BODYH, BODYTRACER)
+// B(i,j) is H = (eta+depth)-hB for h (hB is identically zero) or the tracer.
+template <int SCH, bool TRACER>
+__global__ __launch_bounds__(BX *BY) void k_flux(Grid g, Ptrs p, Flags *fl, const double *B0, double depth, double Dt,
+                                                   double sig2, double *out, int gated) {
+    if (gated && !fl->nonfinite) return;
+    CELL();
+    auto B = [&](int a, int b) -> double {
+        if constexpr (TRACER) return B0[IX(g, a, b)];
+        else return ((B0[IX(g, a, b)]) + (depth));
+    };
+    double dxe = TB(g, SW2_T_DXE, j), dyn = TB(g, SW2_T_DYN, j);
+    double uwm = UW(i - 1, j), uw0 = UW(i, j), vwm = VW(i, j - 1), vw0 = VW(i, j);
+    double CwP = 0.5 * (uwm + fabs(uwm)) * Dt / dxe;
+    double CwM = 0.5 * (uwm - fabs(uwm)) * Dt / dxe;
+    double CeP = 0.5 * (uw0 + fabs(uw0)) * Dt / dxe;
+    double CeM = 0.5 * (uw0 - fabs(uw0)) * Dt / dxe;
+    double CsP = 0.5 * (vwm + fabs(vwm)) * Dt / dyn;
+    double CsM = 0.5 * (vwm - fabs(vwm)) * Dt / dyn;
+    double CnP = 0.5 * (vw0 + fabs(vw0)) * Dt / dyn;
+    double CnM = 0.5 * (vw0 - fabs(vw0)) * Dt / dyn;
+    double bxm2 = B(i - 2, j), bxm1 = B(i - 1, j), b00 = B(i, j), bxp1 = B(i + 1, j), bxp2 = B(i + 2, j);
+    double bym2 = B(i, j - 2), bym1 = B(i, j - 1), byp1 = B(i, j + 1), byp2 = B(i, j + 2);
+    double rxPprev = r2rP(bxm2, bxm1, b00);
+    double rxMprev = r2rM(bxm1, b00, bxp1);
+    double rxP = r2rP(bxm1, b00, bxp1);
+    double rxM = r2rM(b00, bxp1, bxp2);
+    double ryPprev = r2rP(bym2, bym1, b00);
+    double ryMprev = r2rM(bym1, b00, byp1);
+    double ryP = r2rP(bym1, b00, byp1);
+    double ryM = r2rM(b00, byp1, byp2);
+    double BwP = b2bp<SCH>(bxm1, b00, rxPprev, CwP);
+    double BwM = b2bm<SCH>(bxm1, b00, rxMprev, CwM);
+    double BeP = b2bp<SCH>(b00, bxp1, rxP, CeP);
+    double BeM = b2bm<SCH>(b00, bxp1, rxM, CeM);
+    double BsP = b2bp<SCH>(bym1, b00, ryPprev, CsP);
+    double BsM = b2bm<SCH>(bym1, b00, ryMprev, CsM);
+    double BnP = b2bp<SCH>(b00, byp1, ryP, CnP);
+    double BnM = b2bm<SCH>(b00, byp1, ryM, CnM);
+    double d = CwP * BwP + CwM * BwM - CeP * BeP - CeM * BeM + CsP * BsP + CsM * BsM - CnP * BnP - CnM * BnM;
+    if constexpr (TRACER) {
+        double t0 = B0[IX(g, i, j)];
+        double pv2 = Dt * t0 * (uw0 - uwm) / dxe;
+        pv2 += Dt * t0 * (vw0 - vwm) / dyn;
+        d = d + pv2;
+        d -= sig2 * t0;
+    } else {
+        d -= sig2 * B0[IX(g, i, j)];
+    }
+    out[IX(g, i, j)] = d;
+}
+
+// ------------------------------------------------ eta: hyperviscosity, AB, geofactor, update
+__global__ __launch_bounds__(BX *BY) void k_eta_update(Grid g, Ptrs p, Flags *fl, double *dn, const double *m1,
+                                                         const double *m2, const double *le, const double *lle,
+                                                         const double *llle, int hv, double c2, double c4, double c6,
+                                                         int k, double Dt, double depth) {
+    CELL();
+    const int q = IX(g, i, j);
+    double d = dn[q];
+    if (hv) {  // calc.c:346-350
+        d += c2 * le[q];
+        d += c4 * (lle ? lle[q] : 0.0);
+        d += c6 * (llle ? llle[q] : 0.0);
+        dn[q] = d;
+    }
+    double nab = ab(k, d, m1[q], m2[q]);
+    double gf = Dt * TB(g, SW2_T_SINC, j) / TB(g, SW2_T_RZC, j);  // calc.c:420
+    nab -= (gf * ((p.eta[q]) + (depth)) * 0.5 * (VW(i, j) + VW(i, j - 1)));
+    p.eta[q] = p.eta[q] + nab;
+}
+
+__global__ __launch_bounds__(BX *BY) void k_tracer_update(Grid g, Flags *fl, double *tr, const double *dn,
+                                                            const double *m1, const double *m2, int k, int gated) {
+    if (gated && !fl->nonfinite) return;
+    CELL();
+    const int q = IX(g, i, j);
+    tr[q] = tr[q] + ab(k, dn[q], m1[q], m2[q]);
+}
+
+// ------------------------------------------------ momentum tendencies (BODYU, BODYV + pressure + drag + hypervisc + AB)
+template <int SCH>
+__global__ __launch_bounds__(BX *BY) void k_uv(Grid g, Ptrs p, Flags *fl, double Dt, double sigma, int hv, double c2,
+                                                 double c4, double c6, const double *Lu, const double *Lv,
+                                                 const double *LLu, const double *LLv, const double *LLLu,
+                                                 const double *LLLv, double *dun, const double *um1, const double *um2,
+                                                 double *dvn, const double *vm1, const double *vm2, double *duab,
+                                                 double *dvab, int k) {
+    CELL();
+    const int q = IX(g, i, j);
+    double du, dv;
+    {  // BODYU, calc.c:168-200
+        double dxc = TB(g, SW2_T_DXC, j), dyv = TB(g, SW2_T_DYV, j), dxe = TB(g, SW2_T_DXE, j);
+        double a0 = UW(i - 2, j), a1 = UW(i - 1, j), a2 = UW(i, j), a3 = UW(i + 1, j), a4 = UW(i + 2, j);
+        double c0 = UW(i, j - 2), c1 = UW(i, j - 1), c3 = UW(i, j + 1), c4_ = UW(i, j + 2);
+        double vs0 = VW(i, j - 1), vs1 = VW(i + 1, j - 1), vn0 = VW(i, j), vn1 = VW(i + 1, j);
+        double CwP = 0.5 * (0.5 * (a1 + fabs(a1)) + 0.5 * (a2 + fabs(a2))) * Dt / dxc;
+        double CwM = 0.5 * (0.5 * (a1 - fabs(a1)) + 0.5 * (a2 - fabs(a2))) * Dt / dxc;
+        double CeP = 0.5 * (0.5 * (a2 + fabs(a2)) + 0.5 * (a3 + fabs(a3))) * Dt / dxc;
+        double CeM = 0.5 * (0.5 * (a2 - fabs(a2)) + 0.5 * (a3 - fabs(a3))) * Dt / dxc;
+        double CsP = 0.5 * (0.5 * (vs0 + fabs(vs0)) + 0.5 * (vs1 + fabs(vs1))) * Dt / dyv;
+        double CsM = 0.5 * (0.5 * (vs0 - fabs(vs0)) + 0.5 * (vs1 - fabs(vs1))) * Dt / dyv;
+        double CnP = 0.5 * (0.5 * (vn0 + fabs(vn0)) + 0.5 * (vn1 + fabs(vn1))) * Dt / dyv;
+        double CnM = 0.5 * (0.5 * (vn0 - fabs(vn0)) + 0.5 * (vn1 - fabs(vn1))) * Dt / dyv;
+        double rxPprev = r2rP(a0, a1, a2), rxMprev = r2rM(a1, a2, a3), rxP = r2rP(a1, a2, a3), rxM = r2rM(a2, a3, a4);
+        double ryPprev = r2rP(c0, c1, a2), ryMprev = r2rM(c1, a2, c3), ryP = r2rP(c1, a2, c3), ryM = r2rM(a2, c3, c4_);
+        double BwP = b2bp<SCH>(a1, a2, rxPprev, CwP), BwM = b2bm<SCH>(a1, a2, rxMprev, CwM);
+        double BeP = b2bp<SCH>(a2, a3, rxP, CeP), BeM = b2bm<SCH>(a2, a3, rxM, CeM);
+        double BsP = b2bp<SCH>(c1, a2, ryPprev, CsP), BsM = b2bm<SCH>(c1, a2, ryMprev, CsM);
+        double BnP = b2bp<SCH>(a2, c3, ryP, CnP), BnM = b2bm<SCH>(a2, c3, ryM, CnM);
+        double p1 = CwP * BwP + CwM * BwM - CeP * BeP - CeM * BeM + CsP * BsP + CsM * BsM - CnP * BnP - CnM * BnM;
+        double p2 = Dt * (a2 * (a3 - a1) / (dxe + dxe) + a2 * ((vn0 + vn1) / 2. - (vs0 + vs1) / 2.) / dyv);
+        double adv = p1 + p2;
+        double pres = -TB(g, SW2_T_GE, j) * Dt * (p.eta[IX(g, i + 1, j)] - p.eta[q]) / dxc;  // calc.c:517
+        du = adv + pres;                                                                      // core.c:224
+    }
+    {  // BODYV, calc.c:205-237
+        double dxv = TB(g, SW2_T_DXV, j), dyc = TB(g, SW2_T_DYC, j);
+        double dyn2 = TB(g, SW2_T_DYN, j) + TB(g, SW2_T_DYN, j + 1);
+        double uw_m1j1 = UW(i - 1, j + 1), uw_m1j = UW(i - 1, j), uw_j1 = UW(i, j + 1), uw_j = UW(i, j);
+        double a0 = VW(i - 2, j), a1 = VW(i - 1, j), a2 = VW(i, j), a3 = VW(i + 1, j), a4 = VW(i + 2, j);
+        double c0 = VW(i, j - 2), c1 = VW(i, j - 1), c3 = VW(i, j + 1), c4_ = VW(i, j + 2);
+        double CwP = 0.5 * (0.5 * (uw_m1j1 + fabs(uw_m1j1)) + 0.5 * (uw_m1j + fabs(uw_m1j))) * Dt / dxv;
+        double CwM = 0.5 * (0.5 * (uw_m1j1 - fabs(uw_m1j1)) + 0.5 * (uw_m1j - fabs(uw_m1j))) * Dt / dxv;
+        double CeP = 0.5 * (0.5 * (uw_j1 + fabs(uw_j1)) + 0.5 * (uw_j + fabs(uw_j))) * Dt / dxv;
+        double CeM = 0.5 * (0.5 * (uw_j1 - fabs(uw_j1)) + 0.5 * (uw_j - fabs(uw_j))) * Dt / dxv;
+        double CsP = 0.5 * (0.5 * (c1 + fabs(c1)) + 0.5 * (a2 + fabs(a2))) * Dt / dyc;
+        double CsM = 0.5 * (0.5 * (c1 - fabs(c1)) + 0.5 * (a2 - fabs(a2))) * Dt / dyc;
+        double CnP = 0.5 * (0.5 * (a2 + fabs(a2)) + 0.5 * (c3 + fabs(c3))) * Dt / dyc;
+        double CnM = 0.5 * (0.5 * (a2 - fabs(a2)) + 0.5 * (c3 - fabs(c3))) * Dt / dyc;
+        double rxPprev = r2rP(a0, a1, a2), rxMprev = r2rM(a1, a2, a3), rxP = r2rP(a1, a2, a3), rxM = r2rM(a2, a3, a4);
+        double ryPprev = r2rP(c0, c1, a2), ryMprev = r2rM(c1, a2, c3), ryP = r2rP(c1, a2, c3), ryM = r2rM(a2, c3, c4_);
+        double BwP = b2bp<SCH>(a1, a2, rxPprev, CwP), BwM = b2bm<SCH>(a1, a2, rxMprev, CwM);
+        double BeP = b2bp<SCH>(a2, a3, rxP, CeP), BeM = b2bm<SCH>(a2, a3, rxM, CeM);
+        double BsP = b2bp<SCH>(c1, a2, ryPprev, CsP), BsM = b2bm<SCH>(c1, a2, ryMprev, CsM);
+        double BnP = b2bp<SCH>(a2, c3, ryP, CnP), BnM = b2bm<SCH>(a2, c3, ryM, CnM);
+        double p1 = CwP * BwP + CwM * BwM - CeP * BeP - CeM * BeM + CsP * BsP + CsM * BsM - CnP * BnP - CnM * BnM;
+        double p2 = Dt * (a2 * ((uw_j + uw_j1) / 2. - (uw_m1j + uw_m1j1) / 2.) / dxv + a2 * (c3 - c1) / dyn2);
+        double adv = p1 + p2;
+        double pres = -TB(g, SW2_T_GN, j) * Dt * (p.eta[IX(g, i, j + 1)] - p.eta[q]) / TB(g, SW2_T_DYC, j + 1);  // calc.c:538
+        dv = adv + pres;
+    }
+    du -= p.u[q] * sigma * Dt;  // calc.c:396-397
+    dv -= p.v[q] * sigma * Dt;
+    if (hv) {  // core.c:258-266
+        du += c2 * Lu[q];
+        du += c4 * (LLu ? LLu[q] : 0.0);
+        du += c6 * (LLLu ? LLLu[q] : 0.0);
+        dv += c2 * Lv[q];
+        dv += c4 * (LLv ? LLv[q] : 0.0);
+        dv += c6 * (LLLv ? LLLv[q] : 0.0);
+    }
+    dun[q] = du;
+    dvn[q] = dv;
+    duab[q] = ab(k, du, um1[q], um2[q]);
+    dvab[q] = ab(k, dv, vm1[q], vm2[q]);
+}
+
+// ------------------------------------------------ Coriolis (calc.c:542-598), in place
+__global__ __launch_bounds__(BX *BY) void k_cor_u(Grid g, Ptrs p, Flags *fl, const double *duab, double Dt) {
+    CELL();
+    const int q = IX(g, i, j);
+    double semiU = 0.25 * (p.v[q] + p.v[IX(g, i + 1, j)] + p.v[IX(g, i, j - 1)] + p.v[IX(g, i + 1, j - 1)]);
+    double U = p.u[q];
+    double Alpha = Dt * TB(g, SW2_T_FE, j) + (Dt * U * TB(g, SW2_T_SINC, j)) / TB(g, SW2_T_RZC, j);
+    double Beta = 0.25 * Alpha * Alpha;
+    p.u[q] = (U + duab[q] - Beta * U + Alpha * semiU) / (1.0 + Beta);
+}
+
+__global__ __launch_bounds__(BX *BY) void k_cor_v(Grid g, Ptrs p, Flags *fl, const double *dvab, double Dt) {
+    CELL();
+    const int q = IX(g, i, j);
+    // U interior already holds the new values, its halo the previous ones
+    double semiV = 0.25 * (p.u[IX(g, i - 1, j + 1)] + p.u[IX(g, i, j + 1)] + p.u[q] + p.u[IX(g, i - 1, j)]);
+    double V = p.v[q];
+    double Alpha = Dt * TB(g, SW2_T_FN, j) + (Dt * semiV * TB(g, SW2_T_SINN, j)) / TB(g, SW2_T_RZN, j);
+    double Beta = 0.25 * Alpha * Alpha;
+    p.v[q] = (V + dvab[q] - Beta * V - Alpha * semiV) / (1.0 + Beta);
+}
+
+// ------------------------------------------------ Laplacians (laplacian.c:51-151)
+__global__ __launch_bounds__(BX *BY) void k_lapv_zd(Grid g, Flags *fl, const double *uin, const double *vin, double *ze,
+                                                      double *di) {
+    CELL();
+    const int q = IX(g, i, j);
+    double a = (1.0 / TB(g, SW2_T_DXV, j)), b = -(1.0 / TB(g, SW2_T_DYC, j + 1));
+    ze[q] = a * ((vin[IX(g, i + 1, j)] - vin[q]) +
+                 b * (TB(g, SW2_T_DXC, j + 1) * uin[IX(g, i, j + 1)] - TB(g, SW2_T_DXC, j) * uin[q]));
+    a = (1.0 / TB(g, SW2_T_DXE, j));
+    b = (1.0 / TB(g, SW2_T_DYN, j));
+    di[q] = a * ((uin[q] - uin[IX(g, i - 1, j)]) + b * (TB(g, SW2_T_DXV, j) * vin[q] - TB(g, SW2_T_DXV, j - 1) * vin[IX(g, i, j - 1)]));
+}
+
+__global__ void k_lapv_fix1(Grid g, Flags *fl, const double *uin, double *ze, double *di) {
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (fl->fail_step || i > g.nx) return;
+    di[IX(g, i, g.ny + 1)] = (uin[IX(g, i, g.ny)] - uin[IX(g, i - 1, g.ny)]) / TB(g, SW2_T_DXE, g.ny);
+    ze[IX(g, i, 0)] = 0;
+}
+
+__global__ __launch_bounds__(BX *BY) void k_lapv_out(Grid g, Flags *fl, const double *ze, const double *di, double *uo,
+                                                       double *vo) {
+    CELL();
+    const int q = IX(g, i, j);
+    double a = -(1.0 / TB(g, SW2_T_DYN, j)), b = (1.0 / TB(g, SW2_T_DXC, j));
+    uo[q] = a * (ze[q] - ze[IX(g, i, j - 1)]) + b * (di[IX(g, i + 1, j)] - di[q]);
+    a = (1.0 / TB(g, SW2_T_DXV, j));
+    b = (1.0 / TB(g, SW2_T_DYC, j + 1));
+    vo[q] = a * (ze[q] - ze[IX(g, i - 1, j)]) + b * (di[IX(g, i, j + 1)] - di[q]);
+}
+
+__global__ void k_lapv_fix2(Grid g, Flags *fl, double *uo, double *vo) {
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (fl->fail_step || i > g.nx) return;
+    uo[IX(g, i, g.ny + 1)] = uo[IX(g, i, g.ny)];
+    vo[IX(g, i, g.ny + 1)] = 0;
+    uo[IX(g, i, 0)] = uo[IX(g, i, 1)];
+    vo[IX(g, i, 0)] = 0;
+}
+
+__global__ __launch_bounds__(BX *BY) void k_lape_px(Grid g, Flags *fl, const double *ein, double *psi_, double *xi) {
+    CELL();
+    const int q = IX(g, i, j);
+    double fp = TB(g, SW2_T_RMC, j) / TB(g, SW2_T_RZC, j), fx = TB(g, SW2_T_RZN, j) / TB(g, SW2_T_RMN, j);
+    psi_[q] = fp * (ein[IX(g, i + 1, j)] - ein[q]) / (TB(g, SW2_T_DXC, j));
+    xi[q] = fx * (ein[IX(g, i, j + 1)] - ein[q]) / (TB(g, SW2_T_DYC, j));
+}
+
+__global__ __launch_bounds__(BX *BY) void k_lape_out(Grid g, Flags *fl, const double *psi_, const double *xi, double *eo) {
+    CELL();
+    const int q = IX(g, i, j);
+    double kk = 1.0 / (TB(g, SW2_T_RMC, j) * TB(g, SW2_T_RZC, j));
+    eo[q] = kk * ((psi_[q] - psi_[IX(g, i - 1, j)]) / (TB(g, SW2_T_DXE, j)) + (xi[q] - xi[IX(g, i, j - 1)]) / (TB(g, SW2_T_DYN, j)));
+}
+
+__global__ void k_lape_fix(Grid g, Flags *fl, double *eo) {
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (fl->fail_step || i > g.nx) return;
+    eo[IX(g, i, g.ny + 1)] = eo[IX(g, i, g.ny)];
+    eo[IX(g, i, g.ny + 2)] = eo[IX(g, i, g.ny + 1)];
+    eo[IX(g, i, 0)] = eo[IX(g, i, 1)];
+}
+
+// ------------------------------------------------ halo rules
+__device__ __forceinline__ int wrap(int a, int n) { return a < 1 ? a + n : (a > n ? a - n : a); }
+
+// single-map halo_update: every halo cell takes the value of its periodic
+// image in x and y (y first for interior columns, then x for all rows)
+__global__ void k_halo(Grid g, Flags *fl, double *f, int gated) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (fl->fail_step || (gated && !fl->nonfinite)) return;
+    const int W = g.nx + 4;
+    int i, j;
+    if (t < 4 * W) {
+        int r = t / W;
+        i = (t % W) - 1;
+        j = r < 2 ? r - 1 : g.ny + (r - 1);
+    } else {
+        int t2 = t - 4 * W;
+        if (t2 >= 4 * g.ny) return;
+        j = 1 + t2 / 4;
+        int c = t2 % 4;
+        i = c < 2 ? c - 1 : g.nx + (c - 1);
+    }
+    f[IX(g, i, j)] = f[IX(g, wrap(i, g.nx), wrap(j, g.ny))];
+}
+
+// do_channel for u and v (windx, windy are fixed after the first call)
+__global__ void k_channel(Grid g, Flags *fl, double *u, double *v) {
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (fl->fail_step || i > g.nx) return;
+    const int ny = g.ny;
+    v[IX(g, i, 0)] = 0;
+    v[IX(g, i, -1)] = 0;
+    u[IX(g, i, 0)] = u[IX(g, i, 1)];
+    u[IX(g, i, -1)] = u[IX(g, i, 1)];
+    v[IX(g, i, ny)] = 0;
+    v[IX(g, i, ny + 1)] = 0;
+    v[IX(g, i, ny + 2)] = 0;
+    u[IX(g, i, ny + 1)] = u[IX(g, i, ny)];
+    u[IX(g, i, ny + 2)] = u[IX(g, i, ny)];
+}
+
+// ------------------------------------------------ check_stats_scaf guard (sppde_core.c:377-395)
+// SW2 crashes when min or max of the interior is outside [-1e10, 1e10] or not
+// finite.  Its min/max skip NaNs unless the first cell is NaN; hence: fail if
+// any non-NaN value is outside the range, or the cell (1,1) is NaN.
+__global__ __launch_bounds__(BX *BY) void k_check(Grid g, Flags *fl, const double *f, int n, int kind, int gated) {
+    if (gated && !fl->nonfinite) return;
+    CELL();
+    double x = f[IX(g, i, j)];
+    if ((kind == 4 || kind == 5) && !isfinite(x)) fl->nonfinite = 1;
+    bool bad = (x < -1e10) || (x > 1e10) || (i == 1 && j == 1 && x != x);
+    if (bad) {
+        if (atomicCAS(&fl->fail_step, 0, n) == 0) {
+            fl->fail_kind = kind;
+            fl->fail_val = x;
+        }
+    }
+}
+
+// ------------------------------------------------ in-loop perturbations
+__global__ void k_add(Grid g, Flags *fl, double *f, const double *inc) {
+    const int i = 1 + blockIdx.x * BX + threadIdx.x;
+    const int j = 1 + blockIdx.y * BY + threadIdx.y;
+    if (fl->fail_step || i > g.nx || j > g.ny) return;
+    const int q = IX(g, i, j);
+    f[q] += inc[q];
+}
+
+__global__ void k_assign_masked(int S, Flags *fl, double *tr, double *pert, const double *val, const unsigned char *mask) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (fl->fail_step || q >= S) return;
+    if (mask[q]) {
+        tr[q] = val[q];
+        pert[q] = 1.0;
+    }
+}
+
+__global__ void k_zero(int S, double *f) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < S) f[q] = 0;
+}
+
+}  // namespace
+
+// ====================================================================== session
+struct sw2_session {
+    sw *SW;
+    Grid g;
+    int S, form, sch, hv, tracer_active, nstart, n, perturbs_dirty;
+    double t, Dt, depth, sigma, tracer_dis, courant, rest_x, rest_y;
+    double c2, c4, c6;
+    double *d_tab;
+    double *eta, *u, *v, *tr, *pert, *wx, *wy;
+    double *Deta[3], *Du[3], *Dv[3], *Dtr[3];
+    double *duab, *dvab;
+    double *Lu, *Lv, *LLu, *LLv, *LLLu, *LLLv, *ze, *di, *le, *lle, *llle, *psi, *xi;
+    double *pinc, *ptr, *pdu, *pdv;
+    unsigned char *pmask;
+    Flags *fl;
+    hipStream_t st;
+    // host scratch for perturbations
+    double *h_inc[8], *h_trv[8], *h_dv[3][8];
+    unsigned char *h_mask[8];
+};
+
+static int dalloc(double **p, size_t n) {
+    HIPCHECK(hipMalloc((void **)p, n * sizeof(double)));
+    HIPCHECK(hipMemset(*p, 0, n * sizeof(double)));
+    return 0;
+}
+
+static dim3 cells_grid(const Grid &g) { return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY); }
+static dim3 cells_block() { return dim3(BX, BY); }
+
+static void rot(double *h[3]) {
+    double *m2 = h[2];
+    h[2] = h[1];
+    h[1] = h[0];
+    h[0] = m2;
+}
+
+extern "C" int sw2_session_create(sw *SW, int form, sw2_session **out) {
+    *out = NULL;
+    if (form != SW2_FORM_FP64) {
+        sw2_set_error("form %d is not available in this session type", form);
+        return -1;
+    }
+    if (SW->polar != 0) {
+        sw2_set_error("only the channel geometry (polar 0) runs on the GPU");
+        return -1;
+    }
+    if (!sw2_all_zero(SW->hB, SW->M->S)) {
+        sw2_set_error("hB must be identically zero (SW2 never sets it)");
+        return -1;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        sw2_set_error("no HIP device available");
+        return -1;
+    }
+    sw2_session *s = (sw2_session *)calloc(1, sizeof(sw2_session));
+    s->SW = SW;
+    s->S = SW->M->S;
+    s->sch = SW->sch;
+    s->form = form;
+    s->Dt = SW->Dt;
+    s->depth = SW->depth;
+    s->sigma = SW->sigma_dis;
+    s->tracer_dis = SW->tracer_dis;
+    s->courant = SW->Courant_max;
+    s->hv = (SW->nu2 != 0 || SW->nu4 != 0 || SW->nu6 != 0);
+    s->c2 = (SW->nu2) * (SW->Dt);
+    s->c4 = (SW->nu4) * (SW->Dt);
+    s->c6 = (SW->nu6) * (SW->Dt);
+    sw2_cfl_rest(SW, &s->rest_x, &s->rest_y);
+    int inject = 0;
+    for (int k = 0; k < SW->num_Gaussians; k++)
+        if (SW->Gaussians[k * N_COLS_GAUSS + 11] != 0) inject = 1;
+    s->tracer_active = inject || !sw2_all_zero(SW->tracer, s->S);
+    s->g.nx = SW->nx;
+    s->g.ny = SW->ny;
+    s->g.sx = SW->nx + 4;
+    s->g.R = SW->ny + 4;
+    size_t S = (size_t)s->S;
+    int rc = 0;
+    double *tab = (double *)malloc(sizeof(double) * SW2_NT * s->g.R);
+    sw2_row_tables(SW, tab);
+    rc |= dalloc(&s->d_tab, (size_t)SW2_NT * s->g.R);
+    if (!rc && hipMemcpy(s->d_tab, tab, sizeof(double) * SW2_NT * s->g.R, hipMemcpyHostToDevice) != hipSuccess) rc = -1;
+    free(tab);
+    s->g.T = s->d_tab;
+    double **fields[] = {&s->eta, &s->u, &s->v, &s->tr, &s->pert, &s->wx, &s->wy, &s->duab, &s->dvab};
+    for (auto f : fields) rc |= dalloc(f, S);
+    for (int k = 0; k < 3; k++) {
+        rc |= dalloc(&s->Deta[k], S);
+        rc |= dalloc(&s->Du[k], S);
+        rc |= dalloc(&s->Dv[k], S);
+        rc |= dalloc(&s->Dtr[k], S);
+    }
+    if (s->hv) {
+        double **hvf[] = {&s->Lu, &s->Lv, &s->ze, &s->di, &s->le, &s->psi, &s->xi};
+        for (auto f : hvf) rc |= dalloc(f, S);
+        if (SW->nu4 != 0 || SW->nu6 != 0) {
+            rc |= dalloc(&s->LLu, S);
+            rc |= dalloc(&s->LLv, S);
+            rc |= dalloc(&s->lle, S);
+        }
+        if (SW->nu6 != 0) {
+            rc |= dalloc(&s->LLLu, S);
+            rc |= dalloc(&s->LLLv, S);
+            rc |= dalloc(&s->llle, S);
+        }
+    }
+    if (!rc && hipMalloc((void **)&s->fl, sizeof(Flags)) != hipSuccess) rc = -1;
+    if (!rc && hipMemset(s->fl, 0, sizeof(Flags)) != hipSuccess) rc = -1;
+    if (!rc && hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) rc = -1;
+    if (rc) {
+        if (!sw2_last_error()[0]) sw2_set_error("device allocation failed");
+        sw2_session_destroy(s);
+        return -1;
+    }
+    struct {
+        double *d;
+        double *h;
+    } up[] = {{s->eta, SW->eta}, {s->u, SW->u}, {s->v, SW->v}, {s->tr, SW->tracer}, {s->pert, SW->perturbs}, {s->wx, SW->windx}, {s->wy, SW->windy}};
+    for (auto &x : up)
+        if (hipMemcpy(x.d, x.h, sizeof(double) * S, hipMemcpyHostToDevice) != hipSuccess) {
+            sw2_set_error("upload failed");
+            sw2_session_destroy(s);
+            return -1;
+        }
+    s->perturbs_dirty = !sw2_all_zero(SW->perturbs, s->S);
+    s->t = SW->t0;
+    *out = s;
+    return 0;
+}
+
+extern "C" int sw2_session_begin(sw2_session *s, int nstart, double t) {
+    s->nstart = nstart;
+    s->n = nstart;
+    s->t = t;
+    HIPCHECK(hipMemsetAsync(s->fl, 0, sizeof(Flags), s->st));
+    HIPCHECK(hipStreamSynchronize(s->st));
+    return 0;
+}
+
+static void launch_halo(sw2_session *s, double *f, int gated = 0) {
+    int cnt = 4 * (s->g.nx + 4) + 4 * s->g.ny;
+    k_halo<<<(cnt + 255) / 256, 256, 0, s->st>>>(s->g, s->fl, f, gated);
+}
+
+template <int SCH>
+static void launch_flux(sw2_session *s, Ptrs p, bool tracer, const double *B, double sig2, double *out) {
+    if (tracer)
+        k_flux<SCH, true><<<cells_grid(s->g), cells_block(), 0, s->st>>>(s->g, p, s->fl, B, s->depth, s->Dt, sig2, out,
+                                                                         !s->tracer_active);
+    else k_flux<SCH, false><<<cells_grid(s->g), cells_block(), 0, s->st>>>(s->g, p, s->fl, B, s->depth, s->Dt, sig2, out, 0);
+}
+
+template <int SCH>
+static void launch_uv(sw2_session *s, Ptrs p, int k) {
+    k_uv<SCH><<<cells_grid(s->g), cells_block(), 0, s->st>>>(s->g, p, s->fl, s->Dt, s->sigma, s->hv, s->c2, s->c4, s->c6, s->Lu, s->Lv,
+                                                             s->LLu, s->LLv, s->LLLu, s->LLLv, s->Du[0], s->Du[1], s->Du[2], s->Dv[0],
+                                                             s->Dv[1], s->Dv[2], s->duab, s->dvab, k);
+}
+
+#define SCH_SWITCH(F, ...)                 \
+    switch (s->sch) {                      \
+    case 0: F<0>(__VA_ARGS__); break;      \
+    case 1: F<1>(__VA_ARGS__); break;      \
+    case 2: F<2>(__VA_ARGS__); break;      \
+    case 3: F<3>(__VA_ARGS__); break;      \
+    case 4: F<4>(__VA_ARGS__); break;      \
+    case 5: F<5>(__VA_ARGS__); break;      \
+    case 6: F<6>(__VA_ARGS__); break;      \
+    case 7: F<7>(__VA_ARGS__); break;      \
+    case 8: F<8>(__VA_ARGS__); break;      \
+    default: F<9>(__VA_ARGS__); break;     \
+    }
+
+static void lapv(sw2_session *s, const double *uin, const double *vin, double *uo, double *vo) {
+    dim3 G = cells_grid(s->g), B = cells_block();
+    int r = (s->g.nx + 255) / 256;
+    k_lapv_zd<<<G, B, 0, s->st>>>(s->g, s->fl, uin, vin, s->ze, s->di);
+    launch_halo(s, s->ze);
+    launch_halo(s, s->di);
+    k_lapv_fix1<<<r, 256, 0, s->st>>>(s->g, s->fl, uin, s->ze, s->di);
+    k_lapv_out<<<G, B, 0, s->st>>>(s->g, s->fl, s->ze, s->di, uo, vo);
+    launch_halo(s, uo);
+    launch_halo(s, vo);
+    k_lapv_fix2<<<r, 256, 0, s->st>>>(s->g, s->fl, uo, vo);
+}
+
+static void lape(sw2_session *s, const double *ein, double *eo) {
+    dim3 G = cells_grid(s->g), B = cells_block();
+    k_lape_px<<<G, B, 0, s->st>>>(s->g, s->fl, ein, s->psi, s->xi);
+    k_lape_out<<<G, B, 0, s->st>>>(s->g, s->fl, s->psi, s->xi, eo);
+    launch_halo(s, eo);
+    k_lape_fix<<<(s->g.nx + 255) / 256, 256, 0, s->st>>>(s->g, s->fl, eo);
+}
+
+static int host_perturbations(sw2_session *s, int n, double t, int *eta_changed) {
+    sw *SW = s->SW;
+    *eta_changed = 0;
+    if (SW->num_Gaussians == 0 && SW->num_vortices == 0) return 0;
+    size_t S = (size_t)s->S;
+    if (!s->h_inc[0]) {
+        for (int k = 0; k < 8; k++) {
+            s->h_inc[k] = (double *)malloc(sizeof(double) * S);
+            s->h_trv[k] = (double *)malloc(sizeof(double) * S);
+            s->h_mask[k] = (unsigned char *)malloc(S);
+            for (int q = 0; q < 3; q++) s->h_dv[q][k] = (double *)malloc(sizeof(double) * S);
+        }
+        if (dalloc(&s->pinc, S) || dalloc(&s->ptr, S) || dalloc(&s->pdu, S) || dalloc(&s->pdv, S)) return -1;
+        HIPCHECK(hipMalloc((void **)&s->pmask, S));
+    }
+    int K = sw2_gaussians_at(SW, s->Dt, t, 8, s->h_inc, s->h_trv, s->h_mask);
+    if (K < 0) return -1;
+    int V = sw2_vortices_at(SW, s->Dt, t, 8, s->h_dv[0], s->h_dv[1], s->h_dv[2]);
+    if (V < 0) return -1;
+    dim3 G = cells_grid(s->g), B = cells_block();
+    for (int k = 0; k < K; k++) {
+        HIPCHECK(hipMemcpyAsync(s->ptr, s->h_trv[k], sizeof(double) * S, hipMemcpyHostToDevice, s->st));
+        HIPCHECK(hipMemcpyAsync(s->pmask, s->h_mask[k], S, hipMemcpyHostToDevice, s->st));
+        k_assign_masked<<<(s->S + 255) / 256, 256, 0, s->st>>>(s->S, s->fl, s->tr, s->pert, s->ptr, s->pmask);
+        s->perturbs_dirty = 1;
+        if (!isnan(s->h_inc[k][0])) {
+            HIPCHECK(hipMemcpyAsync(s->pinc, s->h_inc[k], sizeof(double) * S, hipMemcpyHostToDevice, s->st));
+            k_add<<<G, B, 0, s->st>>>(s->g, s->fl, s->eta, s->pinc);
+            *eta_changed = 1;
+        }
+        HIPCHECK(hipStreamSynchronize(s->st));  // host scratch is reused
+    }
+    for (int k = 0; k < V; k++) {
+        HIPCHECK(hipMemcpyAsync(s->pinc, s->h_dv[0][k], sizeof(double) * S, hipMemcpyHostToDevice, s->st));
+        HIPCHECK(hipMemcpyAsync(s->pdu, s->h_dv[1][k], sizeof(double) * S, hipMemcpyHostToDevice, s->st));
+        HIPCHECK(hipMemcpyAsync(s->pdv, s->h_dv[2][k], sizeof(double) * S, hipMemcpyHostToDevice, s->st));
+        k_add<<<G, B, 0, s->st>>>(s->g, s->fl, s->eta, s->pinc);
+        k_add<<<G, B, 0, s->st>>>(s->g, s->fl, s->u, s->pdu);
+        k_add<<<G, B, 0, s->st>>>(s->g, s->fl, s->v, s->pdv);
+        launch_halo(s, s->eta);
+        launch_halo(s, s->u);
+        launch_halo(s, s->v);
+        *eta_changed = 1;
+        HIPCHECK(hipStreamSynchronize(s->st));
+    }
+    (void)n;
+    return 0;
+}
+
+static int step_once(sw2_session *s) {
+    const int n = s->n + 1;
+    const int k = n - s->nstart - 1;
+    const double t = s->t + s->Dt;
+    const int slot = n & 1;
+    dim3 G = cells_grid(s->g), B = cells_block();
+    Ptrs p{s->eta, s->u, s->v, s->wx, s->wy, s->tr};
+    k_cfl<<<G, B, 0, s->st>>>(s->g, p, s->fl, slot);
+    k_cfl_fin<<<1, 1, 0, s->st>>>(s->fl, slot, n, s->courant, s->Dt, s->rest_x, s->rest_y);
+    {  // tracer: skipped on the device while it is identically zero and u, v finite
+        const int gated = !s->tracer_active;
+        double sig2 = 2.0 * s->Dt * (s->tracer_dis);
+        SCH_SWITCH(launch_flux, s, p, true, s->tr, sig2, s->Dtr[0]);
+        k_tracer_update<<<G, B, 0, s->st>>>(s->g, s->fl, s->tr, s->Dtr[0], s->Dtr[1], s->Dtr[2], k, gated);
+        rot(s->Dtr);
+        launch_halo(s, s->tr, gated);
+    }
+    double sig2 = 2.0 * s->Dt * (s->sigma);
+    SCH_SWITCH(launch_flux, s, p, false, s->eta, sig2, s->Deta[0]);
+    if (s->hv) {
+        lape(s, s->u, s->le);
+        if (s->lle) {
+            lape(s, s->le, s->lle);
+            if (s->llle) lape(s, s->lle, s->llle);
+        }
+    }
+    k_eta_update<<<G, B, 0, s->st>>>(s->g, p, s->fl, s->Deta[0], s->Deta[1], s->Deta[2], s->le, s->lle, s->llle, s->hv, s->c2,
+                                      s->c4, s->c6, k, s->Dt, s->depth);
+    rot(s->Deta);
+    launch_halo(s, s->eta);
+    if (s->hv) {
+        lapv(s, s->u, s->v, s->Lu, s->Lv);
+        if (s->LLu) {
+            lapv(s, s->Lu, s->Lv, s->LLu, s->LLv);
+            if (s->LLLu) lapv(s, s->LLu, s->LLv, s->LLLu, s->LLLv);
+        }
+    }
+    SCH_SWITCH(launch_uv, s, p, k);
+    rot(s->Du);
+    rot(s->Dv);
+    k_cor_u<<<G, B, 0, s->st>>>(s->g, p, s->fl, s->duab, s->Dt);
+    k_cor_v<<<G, B, 0, s->st>>>(s->g, p, s->fl, s->dvab, s->Dt);
+    launch_halo(s, s->u);
+    launch_halo(s, s->v);
+    if (s->perturbs_dirty) {
+        k_zero<<<(s->S + 255) / 256, 256, 0, s->st>>>(s->S, s->pert);
+        s->perturbs_dirty = 0;
+    }
+    int eta_changed = 0;
+    if (host_perturbations(s, n, t, &eta_changed)) return -1;
+    launch_halo(s, s->tr, !s->tracer_active);
+    if (eta_changed) launch_halo(s, s->eta);
+    k_channel<<<(s->g.nx + 255) / 256, 256, 0, s->st>>>(s->g, s->fl, s->u, s->v);
+    k_check<<<G, B, 0, s->st>>>(s->g, s->fl, s->tr, n, 2, !s->tracer_active);
+    k_check<<<G, B, 0, s->st>>>(s->g, s->fl, s->eta, n, 3, 0);
+    k_check<<<G, B, 0, s->st>>>(s->g, s->fl, s->u, n, 4, 0);
+    k_check<<<G, B, 0, s->st>>>(s->g, s->fl, s->v, n, 5, 0);
+    s->n = n;
+    s->t = t;
+    return 0;
+}
+
+extern "C" int sw2_session_step(sw2_session *s, int nsteps) {
+    for (int q = 0; q < nsteps; q++) {
+        if (step_once(s)) return -1;
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s->st));
+    return 0;
+}
+
+extern "C" int sw2_session_time(sw2_session *s, int nsteps, double *ms) {
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    HIPCHECK(hipEventRecord(e0, s->st));
+    for (int q = 0; q < nsteps; q++)
+        if (step_once(s)) return -1;
+    HIPCHECK(hipEventRecord(e1, s->st));
+    HIPCHECK(hipEventSynchronize(e1));
+    float f = 0;
+    HIPCHECK(hipEventElapsedTime(&f, e0, e1));
+    *ms = f;
+    HIPCHECK(hipEventDestroy(e0));
+    HIPCHECK(hipEventDestroy(e1));
+    return 0;
+}
+
+extern "C" int sw2_session_download(sw2_session *s, sw *SW) {
+    size_t S = (size_t)s->S;
+    HIPCHECK(hipStreamSynchronize(s->st));
+    struct {
+        double *d;
+        double *h;
+    } dn[] = {{s->eta, SW->eta}, {s->u, SW->u}, {s->v, SW->v}, {s->tr, SW->tracer}, {s->pert, SW->perturbs}};
+    for (auto &x : dn) HIPCHECK(hipMemcpy(x.h, x.d, sizeof(double) * S, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int sw2_session_failed_step(sw2_session *s, char *msg, int msglen) {
+    Flags f;
+    if (hipMemcpy(&f, s->fl, sizeof f, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    if (!f.fail_step) return 0;
+    if (msg && msglen > 0) {
+        static const char *names[] = {"", "cfl", "tracer", "eta", "u", "v"};
+        if (f.fail_kind == 1)
+            snprintf(msg, msglen, "Smallest possible timestep is Dt_CFL=%.6e while Dt=%.6e.\n", f.fail_val, s->Dt);
+        else
+            snprintf(msg, msglen, "OOOPS nonsense value detected in %s at step %d (value %e); range acceptable = %e to %e \n",
+                     names[f.fail_kind < 6 ? f.fail_kind : 0], f.fail_step, f.fail_val, -1e10, 1e10);
+    }
+    return f.fail_step;
+}
+
+extern "C" int sw2_session_last_cfl(sw2_session *s, double *dtcfl) {
+    Flags f;
+    HIPCHECK(hipStreamSynchronize(s->st));
+    HIPCHECK(hipMemcpy(&f, s->fl, sizeof f, hipMemcpyDeviceToHost));
+    *dtcfl = f.last_dtcfl;
+    return 0;
+}
+
+extern "C" int sw2_session_state(sw2_session *s, int *n, double *t) {
+    if (n) *n = s->n;
+    if (t) *t = s->t;
+    return 0;
+}
+
+extern "C" void sw2_session_destroy(sw2_session *s) {
+    if (!s) return;
+    if (s->st) hipStreamSynchronize(s->st);
+    double *all[] = {s->d_tab, s->eta, s->u, s->v, s->tr, s->pert, s->wx, s->wy, s->duab, s->dvab, s->Lu, s->Lv, s->LLu, s->LLv,
+                     s->LLLu, s->LLLv, s->ze, s->di, s->le, s->lle, s->llle, s->psi, s->xi, s->pinc, s->ptr, s->pdu, s->pdv};
+    for (auto p : all)
+        if (p) hipFree(p);
+    for (int k = 0; k < 3; k++) {
+        if (s->Deta[k]) hipFree(s->Deta[k]);
+        if (s->Du[k]) hipFree(s->Du[k]);
+        if (s->Dv[k]) hipFree(s->Dv[k]);
+        if (s->Dtr[k]) hipFree(s->Dtr[k]);
+    }
+    if (s->pmask) hipFree(s->pmask);
+    if (s->fl) hipFree(s->fl);
+    if (s->st) hipStreamDestroy(s->st);
+    for (int k = 0; k < 8; k++) {
+        free(s->h_inc[k]);
+        free(s->h_trv[k]);
+        free(s->h_mask[k]);
+        for (int q = 0; q < 3; q++) free(s->h_dv[q][k]);
+    }
+    free(s);
+}
+
+extern "C" const char *sw2_build_info(void) {
+    return "libsw2 gfx950 FP64 -O3 -ffp-contract=off";
+}

@@ -1,0 +1,129 @@
+"""The C-ABI library: it loads, exports every function include/sw2_hip.h
+declares, and its host-side pieces (case reader, expression evaluator, frame
+IO, world set-up) behave as SW2's.  CPU only: no kernel is launched."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import sw2_amd
+from sw2_amd import World, cases
+from sw2_amd._ffi import EXPORTS, LIB_PATH
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    txt = open(os.path.join(ROOT, "include", "sw2_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    txt = re.sub(r"#define[^\n]*", "", txt)
+    names = re.findall(r"\b([A-Za-z_]\w*)\s*\([^;{}]*\)\s*;", txt)
+    return sorted(set(n for n in names if n not in ("if", "while", "sizeof")))
+
+
+def test_library_exports_every_declared_function():
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    decl = declared_functions()
+    assert len(decl) >= 35
+    missing = [n for n in decl if n not in exported]
+    assert not missing, missing
+    assert set(EXPORTS) <= set(decl)
+
+
+def test_library_is_built_for_gfx950():
+    out = subprocess.run(["strings", LIB_PATH], capture_output=True, text=True).stdout
+    assert "gfx950" in out
+    assert b"gfx950" in sw2_amd.lib().sw2_build_info()
+
+
+@pytest.mark.parametrize("expr,val", [
+    ("2+3^2", 11.0), ("-2^2", 4.0), ("2^3^2", 64.0), ("log(100)", 2.0), ("ln(e)", 1.0),
+    ("10 % 4", 2.0), ("pow(2,10)", 1024.0), ("sin 0", 0.0), ("(1,2,3)", 3.0), ("86400*20", 1728000.0),
+    ("12*3600/50", 864.0), ("fac 5", 120.0), ("ncr(5,2)", 10.0), ("atan2(1,1)*4", np.pi), ("--3", 3.0),
+])
+def test_expression_evaluator_tinyexpr_semantics(expr, val):
+    """'^' left-associative with unary minus binding tighter, log = log10
+    (the compile flags of sppde_tinyexpr.c:25-35)."""
+    assert sw2_amd.eval_expr(expr) == pytest.approx(val, rel=1e-15)
+
+
+def test_expression_errors():
+    with pytest.raises(sw2_amd.SW2Error):
+        sw2_amd.eval_expr("2+")
+    with pytest.raises(sw2_amd.SW2Error):
+        sw2_amd.eval_expr("foo(3)")
+
+
+def test_reads_reference_case_file(golden):
+    cwd = os.getcwd()
+    os.chdir(os.path.join(golden, "jupiter"))
+    try:
+        p = sw2_amd.read_case("jupiters.sw")
+    finally:
+        os.chdir(cwd)
+    assert (p.sch, p.polar, p.nx, p.ny) == (1, 0, 50, 50)
+    assert (p.lon0_deg, p.lon1_deg, p.lat0_deg, p.lat1_deg) == (0.0, 90.0, -35.0, -10.0)
+    assert (p.t0, p.t1, p.Dt, p.CFL, p.Tau) == (0.0, 1728000.0, 50.0, 0.25, 1e7)
+    assert list(p.hypernu) == [0.0, 0.0, 0.0]
+    assert (p.IteInfo, p.SaveEvery, p.LoadFrom, p.reftype) == (100, 864, 0, 0)
+    assert (p.Omega, p.rE, p.rP, p.gE, p.depth, p.geoeq) == (0.000176, 7.1492e7, 6.6852e7, 22.58, 1000.0, 0)
+    assert p.num_vortices == 0 and p.num_Gaussians == 1 and p.wind_rows == 2
+    g = np.ctypeslib.as_array(p.Gaussians, shape=(12,))
+    assert list(g) == [0.0, 604800.0, 0.0, 45.0, -22.5, 0.0, -3.6, 4.52e11, 0.0, 3.5e6, 3.5e6, 0.0]
+
+
+def test_case_file_macro_subset(tmp_path):
+    (tmp_path / "inc.txt").write_text("rE 7e7 # radius\nrP 7e7\n")
+    txt = cases.case_text(cases.scaled_params(8, 6, steps=2), "x")[0]
+    txt = "define(NX,8)\ndefine(TWICE,<2*NX>)\n# NX in a comment stays NX\n" + txt.replace("nx 8", "nx NX")
+    txt = txt.replace("include(x.planet)", "Omega 1e-4\ninclude(inc.txt)\ngE 20 dnl deleted with its newline\n")
+    (tmp_path / "m.sw").write_text(txt)
+    p = sw2_amd.read_case(str(tmp_path / "m.sw"))
+    assert p.nx == 8 and p.rE == 7e7 and p.gE == 20.0 and p.Omega == 1e-4
+
+
+def test_case_file_tag_order_is_enforced(tmp_path):
+    txt, planet = cases.case_text(cases.scaled_params(8, 6, steps=2), "x")
+    (tmp_path / "x.planet").write_text(planet)
+    (tmp_path / "bad.sw").write_text(txt.replace("ny 6", "nz 6"))
+    with pytest.raises(sw2_amd.SW2Error, match="expecting 'ny'"):
+        sw2_amd.read_case(str(tmp_path / "bad.sw"))
+
+
+def test_written_case_round_trips(tmp_path):
+    p = cases.scaled_params(20, 12, steps=7, lat=65.0)
+    w, state = cases.make_world(p, cases.single_bump(3))
+    cases.write_case(str(tmp_path), "rt", p, state, str(tmp_path / "out"))
+    q = sw2_amd.read_case(str(tmp_path / "rt.sw"))
+    for f in ("sch", "nx", "ny", "lon0_deg", "lon1_deg", "lat0_deg", "lat1_deg", "t0", "t1", "Dt", "CFL", "Tau",
+              "IteInfo", "SaveEvery", "LoadFrom", "Omega", "rE", "rP", "gE", "depth", "wind_rows"):
+        assert getattr(q, f) == getattr(p, f), f
+    assert list(q.hypernu) == list(p.hypernu)
+    assert np.array_equal(World(q).field("windx"), World(p).field("windx"))
+
+
+def test_frame_io_round_trip(tmp_path):
+    import ctypes
+    p = cases.scaled_params(20, 12, steps=7, lat=60.0)
+    w, (eta, u, v) = cases.make_world(p, cases.single_bump(4))
+    L = sw2_amd.lib()
+    L.save_binary_sw(3, 42, ctypes.c_double(126.0), b"fr", str(tmp_path).encode(), ctypes.byref(w.sw))
+    w2 = World(p)
+    n, t = ctypes.c_int(), ctypes.c_double()
+    assert L.load_binary_sw(3, ctypes.byref(n), ctypes.byref(t), b"fr", str(tmp_path).encode(), ctypes.byref(w2.sw)) == 0
+    assert (n.value, t.value) == (42, 126.0)
+    for f in ("eta", "u", "v", "tracer", "perturbs"):
+        assert np.array_equal(w2.field(f), w.field(f)), f   # interior + halo_update reproduces the halos
+    assert L.load_binary_sw(9, ctypes.byref(n), ctypes.byref(t), b"fr", str(tmp_path).encode(), ctypes.byref(w2.sw)) == 1
+
+
+def test_world_rejects_bad_parameters():
+    with pytest.raises(sw2_amd.SW2Error, match="too small"):
+        World(cases.scaled_params(2, 2, steps=1))
+    with pytest.raises(sw2_amd.SW2Error, match="a la parra"):
+        World(cases.scaled_params(8, 8, steps=1, sch=11))
+    with pytest.raises(sw2_amd.SW2Error, match="out of bounds"):
+        World(cases.scaled_params(8, 8, steps=1, wind=np.array([[-10.0, 0.0], [10.0, 0.0]])))
