@@ -59,6 +59,11 @@ typedef struct {
 #define MAXOUTLEN 1000
 #endif
 
+/* version string the driver prints (sw.h:13 holds SW2's) */
+#ifndef VERSION
+#define VERSION "SW 0.125 (gfx950)"
+#endif
+
 /* ---------------------------------------------------------------------------
  * The world (sw.h:22-95).  Field names, types and meaning are SW2's so that
  * a driver written against sw.h compiles unchanged against this header.
@@ -209,6 +214,9 @@ int load_binary_sw(int frame, int *nstep, double *timedouble, char *bname, char 
 /* Forms of the device step. */
 #define SW2_FORM_FP64  0   /* plain FP64, same results as SW2's CPU path */
 #define SW2_FORM_MIXED 1   /* FP32 anomalies about an FP64 balanced mean */
+/* Flag or'ed into the form: run one kernel per phase of the step even where
+ * the fused two-kernel step applies (same results; for cross-checks). */
+#define SW2_PHASED 0x100
 
 /* A device session owns the device copy of one world.  Creating it uploads
  * eta, u, v, tracer, perturbs, hB, windx, windy (halos included) and the
@@ -231,6 +239,10 @@ int sw2_session_download(sw2_session *s, sw *SW);
 int sw2_session_failed_step(sw2_session *s, char *msg, int msglen);
 /* Dt_CFL of the most recent step (compute_Dt, calc.c:701-738). */
 int sw2_session_last_cfl(sw2_session *s, double *dtcfl);
+/* 1 when the session steps with the two fused kernels, 0 when it runs one
+ * kernel per phase (in-loop storms or vortices, del-6, active tracer, grids
+ * narrower than 128 columns, or SW2_PHASED). */
+int sw2_session_fused(sw2_session *s);
 /* Current step index and time. */
 int sw2_session_state(sw2_session *s, int *n, double *t);
 void sw2_session_destroy(sw2_session *s);

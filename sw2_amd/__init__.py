@@ -176,6 +176,10 @@ class Session:
         _check(lib().sw2_session_last_cfl(self.h, byref(d)), "sw2_session_last_cfl")
         return d.value
 
+    @property
+    def fused(self):
+        return bool(lib().sw2_session_fused(self.h))
+
     def state(self):
         n, t = c_int(), c_double()
         lib().sw2_session_state(self.h, byref(n), byref(t))

@@ -53,6 +53,9 @@ struct Flags {
     // set once u or v holds a non-finite value: from then on an identically
     // zero tracer no longer stays zero in SW2 (0*NaN), so its kernels run
     int nonfinite;
+    // fields whose check failed at fail_step (bit = kind); SW2 checks tracer,
+    // eta, u, v in that order and reports the first (core.c:309-312)
+    int kind_mask;
     double fail_val;
     double last_dtcfl;
 };
@@ -454,18 +457,23 @@ __global__ void k_channel(Grid g, Flags *fl, double *u, double *v) {
 // SW2 crashes when min or max of the interior is outside [-1e10, 1e10] or not
 // finite.  Its min/max skip NaNs unless the first cell is NaN; hence: fail if
 // any non-NaN value is outside the range, or the cell (1,1) is NaN.
+__device__ __forceinline__ void flag_bad(Flags *fl, int n, int kind) {
+    int old = atomicCAS(&fl->fail_step, 0, n);
+    if (old == 0 || old == n) atomicOr(&fl->kind_mask, 1 << kind);
+}
+
+// the checks of one step all run, even after one of them failed: the step
+// completes, as SW2's does before check_stats_scaf aborts
 __global__ __launch_bounds__(BX *BY) void k_check(Grid g, Flags *fl, const double *f, int n, int kind, int gated) {
     if (gated && !fl->nonfinite) return;
-    CELL();
+    const int i = 1 + blockIdx.x * BX + threadIdx.x;
+    const int j = 1 + blockIdx.y * BY + threadIdx.y;
+    if (fl->fail_step && fl->fail_step != n) return;
+    if (i > g.nx || j > g.ny) return;
     double x = f[IX(g, i, j)];
     if ((kind == 4 || kind == 5) && !isfinite(x)) fl->nonfinite = 1;
     bool bad = (x < -1e10) || (x > 1e10) || (i == 1 && j == 1 && x != x);
-    if (bad) {
-        if (atomicCAS(&fl->fail_step, 0, n) == 0) {
-            fl->fail_kind = kind;
-            fl->fail_val = x;
-        }
-    }
+    if (bad) flag_bad(fl, n, kind);
 }
 
 // ------------------------------------------------ in-loop perturbations
@@ -493,11 +501,17 @@ __global__ void k_zero(int S, double *f) {
 
 }  // namespace
 
+#include "sw2_fused.inc"
+
 // ====================================================================== session
 struct sw2_session {
     sw *SW;
     Grid g;
     int S, form, sch, hv, tracer_active, nstart, n, perturbs_dirty;
+    int fused;      // 0: one kernel per phase; 1: the two fused kernels per step
+    int hvmode;     // fused path: 0 none, 2 del-2 and del-4 (del-6 runs phased)
+    int cfl_ready;  // CFL maxima of the next step already reduced
+    double *eta2, *u2, *v2;  // ping-pong partners of eta, u, v on the fused path
     double t, Dt, depth, sigma, tracer_dis, courant, rest_x, rest_y;
     double c2, c4, c6;
     double *d_tab;
@@ -532,6 +546,8 @@ static void rot(double *h[3]) {
 
 extern "C" int sw2_session_create(sw *SW, int form, sw2_session **out) {
     *out = NULL;
+    const int force_phased = (form & SW2_PHASED) != 0;
+    form &= ~SW2_PHASED;
     if (form != SW2_FORM_FP64) {
         sw2_set_error("form %d is not available in this session type", form);
         return -1;
@@ -574,11 +590,39 @@ extern "C" int sw2_session_create(sw *SW, int form, sw2_session **out) {
     s->g.R = SW->ny + 4;
     size_t S = (size_t)s->S;
     int rc = 0;
-    double *tab = (double *)malloc(sizeof(double) * SW2_NT * s->g.R);
+    const int R = s->g.R, NTD = SW2_NT + 2;
+    double *tab = (double *)calloc((size_t)NTD * R, sizeof(double));
     sw2_row_tables(SW, tab);
-    rc |= dalloc(&s->d_tab, (size_t)SW2_NT * s->g.R);
-    if (!rc && hipMemcpy(s->d_tab, tab, sizeof(double) * SW2_NT * s->g.R, hipMemcpyHostToDevice) != hipSuccess) rc = -1;
+    // windx as the fused kernels read it: the row value everywhere except in
+    // the four corner blocks, which keep the y-wrap of the first halo update
+    int wind_rows_ok = 1;
+    for (int j = -1; j <= SW->ny + 2; j++) {
+        tab[T_WROW * R + j + 1] = ac(SW->windx, 1, j, SW->M);
+        tab[T_WCOR * R + j + 1] = ac(SW->windx, 0, j, SW->M);
+        for (int i = -1; i <= SW->nx + 2; i++) {
+            bool corner = (i < 1 || i > SW->nx) && (j < 1 || j > SW->ny);
+            double w = corner ? tab[T_WCOR * R + j + 1] : tab[T_WROW * R + j + 1];
+            if (memcmp(&w, &ac(SW->windx, i, j, SW->M), sizeof w) != 0) wind_rows_ok = 0;
+        }
+    }
+    int windy_zero = 1;
+    for (int q = 0; q < s->S; q++) {
+        unsigned long long b;
+        memcpy(&b, &SW->windy[q], sizeof b);
+        if (b) windy_zero = 0;
+    }
+    s->fused = !force_phased && wind_rows_ok && windy_zero && !s->tracer_active && SW->num_Gaussians == 0 &&
+               SW->num_vortices == 0 && SW->nu6 == 0 && SW->nx >= 2 * FTX && SW->ny >= 4 &&
+               !(SW->nu2 != 0 && SW->nu4 == 0);
+    s->hvmode = s->hv ? 2 : 0;
+    rc |= dalloc(&s->d_tab, (size_t)NTD * R);
+    if (!rc && hipMemcpy(s->d_tab, tab, sizeof(double) * NTD * R, hipMemcpyHostToDevice) != hipSuccess) rc = -1;
     free(tab);
+    if (s->fused) {
+        rc |= dalloc(&s->eta2, S);
+        rc |= dalloc(&s->u2, S);
+        rc |= dalloc(&s->v2, S);
+    }
     s->g.T = s->d_tab;
     double **fields[] = {&s->eta, &s->u, &s->v, &s->tr, &s->pert, &s->wx, &s->wy, &s->duab, &s->dvab};
     for (auto f : fields) rc |= dalloc(f, S);
@@ -628,6 +672,7 @@ extern "C" int sw2_session_create(sw *SW, int form, sw2_session **out) {
 
 extern "C" int sw2_session_begin(sw2_session *s, int nstart, double t) {
     s->nstart = nstart;
+    s->cfl_ready = 0;
     s->n = nstart;
     s->t = t;
     HIPCHECK(hipMemsetAsync(s->fl, 0, sizeof(Flags), s->st));
@@ -739,7 +784,70 @@ static int host_perturbations(sw2_session *s, int n, double t, int *eta_changed)
     return 0;
 }
 
-static int step_once(sw2_session *s) {
+template <int SCH, int HV>
+static void launch_fused(sw2_session *s, int n, int k, int slot) {
+    dim3 G((s->g.nx + FTX - 1) / FTX, (s->g.ny + FTY - 1) / FTY);
+    double sig2 = 2.0 * s->Dt * (s->sigma);
+    kf_eta<SCH, HV><<<G, FNT, 0, s->st>>>(s->g, s->fl, s->eta, s->u, s->v, s->eta2, s->Deta[0], s->Deta[1], s->Deta[2], n, k,
+                                           slot, s->Dt, s->depth, sig2, s->c2, s->c4, s->c6, s->courant, s->rest_x,
+                                           s->rest_y);
+    kf_uv<SCH, HV><<<G, FNT, 0, s->st>>>(s->g, s->fl, s->eta2, s->u, s->v, s->u2, s->v2, s->Du[0], s->Du[1], s->Du[2], s->Dv[0],
+                                          s->Dv[1], s->Dv[2], n, k, slot ^ 1, s->Dt, s->sigma, s->c2, s->c4);
+}
+
+template <int SCH>
+static void launch_fused_hv(sw2_session *s, int n, int k, int slot) {
+    if (s->hvmode) launch_fused<SCH, 2>(s, n, k, slot);
+    else launch_fused<SCH, 0>(s, n, k, slot);
+}
+
+static void swap_ptr(double **a, double **b) {
+    double *t = *a;
+    *a = *b;
+    *b = t;
+}
+
+// the fused step: same order and results as step_phased below
+static int step_fused(sw2_session *s) {
+    const int n = s->n + 1;
+    const int k = n - s->nstart - 1;
+    const double t = s->t + s->Dt;
+    const int slot = n & 1;
+    dim3 G = cells_grid(s->g), B = cells_block();
+    Ptrs p{s->eta, s->u, s->v, s->wx, s->wy, s->tr};
+    if (!s->cfl_ready) {
+        k_cfl<<<G, B, 0, s->st>>>(s->g, p, s->fl, slot);
+        s->cfl_ready = 1;
+    }
+    {  // tracer: runs only once u or v went non-finite (it is identically zero)
+        double sig2 = 2.0 * s->Dt * (s->tracer_dis);
+        SCH_SWITCH(launch_flux, s, p, true, s->tr, sig2, s->Dtr[0]);
+        k_tracer_update<<<G, B, 0, s->st>>>(s->g, s->fl, s->tr, s->Dtr[0], s->Dtr[1], s->Dtr[2], k, 1);
+        rot(s->Dtr);
+        launch_halo(s, s->tr, 1);
+    }
+    SCH_SWITCH(launch_fused_hv, s, n, k, slot);
+    swap_ptr(&s->eta, &s->eta2);
+    swap_ptr(&s->u, &s->u2);
+    swap_ptr(&s->v, &s->v2);
+    rot(s->Deta);
+    rot(s->Du);
+    rot(s->Dv);
+    if (s->perturbs_dirty) {
+        k_zero<<<(s->S + 255) / 256, 256, 0, s->st>>>(s->S, s->pert);
+        s->perturbs_dirty = 0;
+    }
+    k_check<<<G, B, 0, s->st>>>(s->g, s->fl, s->tr, n, 2, 1);
+    s->n = n;
+    s->t = t;
+    return 0;
+}
+
+static int step_phased(sw2_session *s);
+
+static int step_once(sw2_session *s) { return s->fused ? step_fused(s) : step_phased(s); }
+
+static int step_phased(sw2_session *s) {
     const int n = s->n + 1;
     const int k = n - s->nstart - 1;
     const double t = s->t + s->Dt;
@@ -844,11 +952,14 @@ extern "C" int sw2_session_failed_step(sw2_session *s, char *msg, int msglen) {
     if (!f.fail_step) return 0;
     if (msg && msglen > 0) {
         static const char *names[] = {"", "cfl", "tracer", "eta", "u", "v"};
+        int kind = 0;
+        for (int b = 2; b <= 5 && !kind; b++)
+            if (f.kind_mask & (1 << b)) kind = b;
         if (f.fail_kind == 1)
             snprintf(msg, msglen, "Smallest possible timestep is Dt_CFL=%.6e while Dt=%.6e.\n", f.fail_val, s->Dt);
         else
-            snprintf(msg, msglen, "OOOPS nonsense value detected in %s at step %d (value %e); range acceptable = %e to %e \n",
-                     names[f.fail_kind < 6 ? f.fail_kind : 0], f.fail_step, f.fail_val, -1e10, 1e10);
+            snprintf(msg, msglen, "OOOPS nonsense value detected in field <%s> at step %d; range acceptable = %e to %e \n",
+                     names[kind], f.fail_step, -1e10, 1e10);
     }
     return f.fail_step;
 }
@@ -870,7 +981,7 @@ extern "C" int sw2_session_state(sw2_session *s, int *n, double *t) {
 extern "C" void sw2_session_destroy(sw2_session *s) {
     if (!s) return;
     if (s->st) hipStreamSynchronize(s->st);
-    double *all[] = {s->d_tab, s->eta, s->u, s->v, s->tr, s->pert, s->wx, s->wy, s->duab, s->dvab, s->Lu, s->Lv, s->LLu, s->LLv,
+    double *all[] = {s->d_tab, s->eta, s->u, s->v, s->eta2, s->u2, s->v2, s->tr, s->pert, s->wx, s->wy, s->duab, s->dvab, s->Lu, s->Lv, s->LLu, s->LLv,
                      s->LLLu, s->LLLv, s->ze, s->di, s->le, s->lle, s->llle, s->psi, s->xi, s->pinc, s->ptr, s->pdu, s->pdv};
     for (auto p : all)
         if (p) hipFree(p);
@@ -895,3 +1006,5 @@ extern "C" void sw2_session_destroy(sw2_session *s) {
 extern "C" const char *sw2_build_info(void) {
     return "libsw2 gfx950 FP64 -O3 -ffp-contract=off";
 }
+
+extern "C" int sw2_session_fused(sw2_session *s) { return s->fused; }

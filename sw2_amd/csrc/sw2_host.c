@@ -532,8 +532,21 @@ int sw2_init_params(const sw2_params *P, sw *SW) {
 int initialize(int npx, int npy, int force1proc, char *bname, int copyinput, sw *SW) {
     char fname[MAXOUTLEN + 8];
     snprintf(fname, sizeof fname, "%s.sw", bname);
-    if (copyinput == 1 && g_outpath[0]) {
+    if (copyinput == 1 && g_outpath[0]) { /* sppde_parser.c:185-197: copy, then the expanded text */
         char *txt = NULL, err[256];
+        FILE *fi = fopen(fname, "rb");
+        if (fi) {
+            char dst[2 * MAXOUTLEN + 16];
+            snprintf(dst, sizeof dst, "%s/%s", g_outpath, basename(fname));
+            FILE *fo = fopen(dst, "wb");
+            if (fo) {
+                char buf[4096];
+                size_t k;
+                while ((k = fread(buf, 1, sizeof buf, fi)) > 0) fwrite(buf, 1, k, fo);
+                fclose(fo);
+            }
+            fclose(fi);
+        }
         if (sw2_preprocess(fname, &txt, err, sizeof err) == 0) {
             char dst[2 * MAXOUTLEN + 16];
             snprintf(dst, sizeof dst, "%s/%s.prem4", g_outpath, basename(fname));

@@ -127,3 +127,19 @@ def test_world_rejects_bad_parameters():
         World(cases.scaled_params(8, 8, steps=1, sch=11))
     with pytest.raises(sw2_amd.SW2Error, match="out of bounds"):
         World(cases.scaled_params(8, 8, steps=1, wind=np.array([[-10.0, 0.0], [10.0, 0.0]])))
+
+
+def test_sw_driver_usage_and_rank_check(tmp_path, golden):
+    """The SW driver (main/Main.c without MPI) keeps SW2's usage message and
+    CRASH exit status, and refuses a split domain before any GPU work."""
+    exe = os.path.join(ROOT, "sw2_amd", "SW")
+    r = subprocess.run([exe], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 255
+    assert "SW npx npy input_file output_folder" in r.stderr
+    case = os.path.join(golden, "jupiter", "jupiters")
+    r = subprocess.run([exe, "2", "1", case, str(tmp_path / "out")], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 255
+    assert "npx=2 npy=1" in r.stdout
+    assert "use npx=npy=1" in r.stderr
+    assert (tmp_path / "out" / "jupiters.sw.prem4").exists()
+    assert (tmp_path / "out" / "jupiters.sw").read_bytes() == open(case + ".sw", "rb").read()

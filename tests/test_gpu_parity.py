@@ -21,11 +21,14 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("eta", "u", "v", "tracer", "perturbs")
 
 
-def run_both(params, bumps, nsteps):
+PHASED = 0x100  # SW2_PHASED: one kernel per phase instead of the fused step
+
+
+def run_both(params, bumps, nsteps, form=0):
     w, (eta, u, v) = cases.make_world(params, bumps)
     o = Oracle(params)
     o.load(eta, u, v)
-    s = Session(w)
+    s = Session(w, form)
     s.begin(0, params.t0)
     s.step(nsteps)
     fs, msg = s.failed_step()
@@ -165,3 +168,65 @@ def test_config1_grid_first_steps():
     w, o, s, fs, fo = run_both(p, cases.single_bump(1), 3)
     assert fs == fo == 0
     assert_same(w, o, "config1 x3")
+
+
+# ------------------------------------------------------------ fused step
+@pytest.mark.parametrize("sch", list(range(10)))
+def test_fused_all_limiters(sch):
+    p = cases.scaled_params(128, 24, steps=25, lat=70.0, sch=sch)
+    w, o, s, fs, fo = run_both(p, cases.single_bump(sch + 100), 25)
+    assert s.fused
+    assert fs == fo, (fs, fo, o.failmsg())
+    assert_same(w, o, f"fused sch={sch}")
+
+
+@pytest.mark.parametrize("nx,ny,steps", [(128, 4, 6), (130, 70, 12), (200, 33, 9), (257, 64, 5), (320, 9, 14)])
+def test_fused_ragged_grids(nx, ny, steps):
+    p = cases.scaled_params(nx, ny, steps=steps, lat=60.0)
+    w, o, s, fs, fo = run_both(p, cases.single_bump(nx + 7 * ny), steps)
+    assert s.fused
+    assert fs == fo == 0
+    assert_same(w, o, f"fused {nx}x{ny}")
+
+
+@pytest.mark.parametrize("variant", ["no-hypervisc", "drag", "del2+del4", "coarse-wind", "zero-wind"])
+def test_fused_variants(variant):
+    kw = {}
+    if variant == "drag":
+        kw["Tau"] = 4e5
+    if variant == "coarse-wind":
+        kw["wind"] = np.array([[-90.0, 5.0], [-30.0, -20.0], [0.0, 40.0], [45.0, 10.0], [90.0, -3.0]])
+    if variant == "zero-wind":
+        kw["wind"] = cases.ZERO_WIND
+    p = cases.scaled_params(192, 40, steps=20, lat=72.0, **kw)
+    if variant == "no-hypervisc":
+        p.hypernu[1] = 0.0
+    if variant == "del2+del4":
+        p.hypernu[0] = 3e9
+    w, o, s, fs, fo = run_both(p, cases.single_bump(42), 20)
+    assert s.fused
+    assert fs == fo == 0
+    assert_same(w, o, f"fused {variant}")
+
+
+def test_fused_and_phased_paths_agree():
+    p = cases.scaled_params(256, 96, steps=120, lat=75.0)
+    bumps = cases.single_bump(77) + [(180.0, -30.0, 150.0, 6.0)]
+    w1, o, s1, f1, fo = run_both(p, bumps, 120)
+    w2, _, s2, f2, _ = run_both(p, bumps, 120, form=PHASED)
+    assert s1.fused and not s2.fused
+    assert f1 == f2 == fo == 0
+    assert_same(w1, o, "fused")
+    assert_same(w2, o, "phased")
+
+
+def test_fused_guards():
+    p = cases.scaled_params(128, 20, steps=10, lat=70.0, Dt=50000.0)
+    w, o, s, fs, fo = run_both(p, cases.single_bump(7), 10)
+    assert s.fused and fo > 0 and fs == fo
+    assert_same(w, o, "fused frozen at the CFL failure")
+    p = cases.scaled_params(128, 20, steps=5, lat=70.0, CFL=1e9)
+    w, o, s, fs, fo = run_both(p, [(100.0, 10.0, 5e10, 5.0)], 5)
+    assert s.fused and fo == 1 and fs == 1
+    assert "eta" in s.failed_step()[1]
+    assert_same(w, o, "fused frozen at the stats failure")
