@@ -21,6 +21,7 @@
 #include <string.h>
 
 #include "sw2_internal.h"
+#include "sw2_xdiv.h"
 
 #define HIPCHECK(x)                                                                     \
     do {                                                                                \
@@ -58,6 +59,9 @@ struct Flags {
     int kind_mask;
     double fail_val;
     double last_dtcfl;
+    // fused path: nonfinite as it stood when this step began (kf_eta copies
+    // it), so that every block of kf_uv agrees on whether the tracer moves
+    int tracer_on;
 };
 
 // ------------------------------------------------------------ limiters
@@ -69,10 +73,10 @@ __device__ __forceinline__ double psi(double r) {
     if constexpr (SCH == 0) return (DMAX(0, DMIN(DMIN((1 + r) / 2.0, 2.0 * r), 2.0)));
     else if constexpr (SCH == 1) return (DMAX(DMAX(0, DMIN(2.0 * r, 1.0)), DMAX(0, DMIN(r, 2.0))));
     else if constexpr (SCH == 2) return (DMIN(r, 1));
-    else if constexpr (SCH == 3) return ((r + fabs(r)) / (1.0 + r));
-    else if constexpr (SCH == 4) return (DMAX(0, DMIN(2.0 * r, DMIN((1 + 2.0 * r) / 3.0, 2.0))));
-    else if constexpr (SCH == 5) return ((1.5 * ((r * r) + r)) / ((r * r) + r + 1.0));
-    else if constexpr (SCH == 6) return (((r * r) + r) / ((r * r) + 1.0));
+    else if constexpr (SCH == 3) return ddiv((r + fabs(r)), (1.0 + r));
+    else if constexpr (SCH == 4) return (DMAX(0, DMIN(2.0 * r, DMIN(ddiv((1 + 2.0 * r), 3.0), 2.0))));
+    else if constexpr (SCH == 5) return ddiv((1.5 * ((r * r) + r)), ((r * r) + r + 1.0));
+    else if constexpr (SCH == 6) return ddiv(((r * r) + r), ((r * r) + 1.0));
     else if constexpr (SCH == 7) return (DMAX(0, DMIN(DMIN(2.0, 2.0 * r), DMIN(0.25 + 0.75 * r, 0.75 + 0.25 * r))));
     else if constexpr (SCH == 8) return (0);
     else return (1.0);
@@ -80,10 +84,10 @@ __device__ __forceinline__ double psi(double r) {
 
 // calc.c:43-44
 __device__ __forceinline__ double r2rP(double Bprev, double Bcurr, double Bpost) {
-    return (((Bpost - Bcurr) != 0) ? ((Bcurr - Bprev) / (Bpost - Bcurr)) : 0);
+    return (((Bpost - Bcurr) != 0) ? ddiv((Bcurr - Bprev), (Bpost - Bcurr)) : 0);
 }
 __device__ __forceinline__ double r2rM(double Bcurr, double Bpost, double Bnext) {
-    return (((Bpost - Bcurr) != 0) ? ((Bnext - Bpost) / (Bpost - Bcurr)) : 0);
+    return (((Bpost - Bcurr) != 0) ? ddiv((Bnext - Bpost), (Bpost - Bcurr)) : 0);
 }
 // calc.c:48-49
 template <int SCH>
@@ -124,8 +128,8 @@ __global__ __launch_bounds__(BX *BY) void k_cfl(Grid g, Ptrs p, Flags *fl, int s
     if (fl->fail_step) return;
     double mx = 0, my = 0;
     if (i <= g.nx && j <= g.ny) {
-        double ax = fabs(UW(i, j)) / TB(g, SW2_T_DXE, j);
-        double ay = fabs(VW(i, j)) / TB(g, SW2_T_DYN, j);
+        double ax = ddiv(fabs(UW(i, j)), TB(g, SW2_T_DXE, j));
+        double ay = ddiv(fabs(VW(i, j)), TB(g, SW2_T_DYN, j));
         if (ax > mx) mx = ax;  // NaN never wins a comparison, as in SW2
         if (ay > my) my = ay;
     }
@@ -159,7 +163,7 @@ __global__ void k_cfl_fin(Flags *fl, int slot, int n, double courant, double Dt,
         mx = rest_x;
         my = rest_y;
     }
-    double dtcfl = (courant / (mx + my));
+    double dtcfl = ddiv(courant, (mx + my));
     fl->last_dtcfl = dtcfl;
     fl->cfl[2 * (slot ^ 1)] = 0;
     fl->cfl[2 * (slot ^ 1) + 1] = 0;
@@ -183,14 +187,14 @@ __global__ __launch_bounds__(BX *BY) void k_flux(Grid g, Ptrs p, Flags *fl, cons
     };
     double dxe = TB(g, SW2_T_DXE, j), dyn = TB(g, SW2_T_DYN, j);
     double uwm = UW(i - 1, j), uw0 = UW(i, j), vwm = VW(i, j - 1), vw0 = VW(i, j);
-    double CwP = 0.5 * (uwm + fabs(uwm)) * Dt / dxe;
-    double CwM = 0.5 * (uwm - fabs(uwm)) * Dt / dxe;
-    double CeP = 0.5 * (uw0 + fabs(uw0)) * Dt / dxe;
-    double CeM = 0.5 * (uw0 - fabs(uw0)) * Dt / dxe;
-    double CsP = 0.5 * (vwm + fabs(vwm)) * Dt / dyn;
-    double CsM = 0.5 * (vwm - fabs(vwm)) * Dt / dyn;
-    double CnP = 0.5 * (vw0 + fabs(vw0)) * Dt / dyn;
-    double CnM = 0.5 * (vw0 - fabs(vw0)) * Dt / dyn;
+    double CwP = ddiv(0.5 * (uwm + fabs(uwm)) * Dt, dxe);
+    double CwM = ddiv(0.5 * (uwm - fabs(uwm)) * Dt, dxe);
+    double CeP = ddiv(0.5 * (uw0 + fabs(uw0)) * Dt, dxe);
+    double CeM = ddiv(0.5 * (uw0 - fabs(uw0)) * Dt, dxe);
+    double CsP = ddiv(0.5 * (vwm + fabs(vwm)) * Dt, dyn);
+    double CsM = ddiv(0.5 * (vwm - fabs(vwm)) * Dt, dyn);
+    double CnP = ddiv(0.5 * (vw0 + fabs(vw0)) * Dt, dyn);
+    double CnM = ddiv(0.5 * (vw0 - fabs(vw0)) * Dt, dyn);
     double bxm2 = B(i - 2, j), bxm1 = B(i - 1, j), b00 = B(i, j), bxp1 = B(i + 1, j), bxp2 = B(i + 2, j);
     double bym2 = B(i, j - 2), bym1 = B(i, j - 1), byp1 = B(i, j + 1), byp2 = B(i, j + 2);
     double rxPprev = r2rP(bxm2, bxm1, b00);
@@ -212,8 +216,8 @@ __global__ __launch_bounds__(BX *BY) void k_flux(Grid g, Ptrs p, Flags *fl, cons
     double d = CwP * BwP + CwM * BwM - CeP * BeP - CeM * BeM + CsP * BsP + CsM * BsM - CnP * BnP - CnM * BnM;
     if constexpr (TRACER) {
         double t0 = B0[IX(g, i, j)];
-        double pv2 = Dt * t0 * (uw0 - uwm) / dxe;
-        pv2 += Dt * t0 * (vw0 - vwm) / dyn;
+        double pv2 = ddiv(Dt * t0 * (uw0 - uwm), dxe);
+        pv2 += ddiv(Dt * t0 * (vw0 - vwm), dyn);
         d = d + pv2;
         d -= sig2 * t0;
     } else {
@@ -237,7 +241,7 @@ __global__ __launch_bounds__(BX *BY) void k_eta_update(Grid g, Ptrs p, Flags *fl
         dn[q] = d;
     }
     double nab = ab(k, d, m1[q], m2[q]);
-    double gf = Dt * TB(g, SW2_T_SINC, j) / TB(g, SW2_T_RZC, j);  // calc.c:420
+    double gf = TB(g, SW2_T_GF, j);  // Dt*sin(latc)/rZ(latc), calc.c:420
     nab -= (gf * ((p.eta[q]) + (depth)) * 0.5 * (VW(i, j) + VW(i, j - 1)));
     p.eta[q] = p.eta[q] + nab;
 }
@@ -266,14 +270,14 @@ __global__ __launch_bounds__(BX *BY) void k_uv(Grid g, Ptrs p, Flags *fl, double
         double a0 = UW(i - 2, j), a1 = UW(i - 1, j), a2 = UW(i, j), a3 = UW(i + 1, j), a4 = UW(i + 2, j);
         double c0 = UW(i, j - 2), c1 = UW(i, j - 1), c3 = UW(i, j + 1), c4_ = UW(i, j + 2);
         double vs0 = VW(i, j - 1), vs1 = VW(i + 1, j - 1), vn0 = VW(i, j), vn1 = VW(i + 1, j);
-        double CwP = 0.5 * (0.5 * (a1 + fabs(a1)) + 0.5 * (a2 + fabs(a2))) * Dt / dxc;
-        double CwM = 0.5 * (0.5 * (a1 - fabs(a1)) + 0.5 * (a2 - fabs(a2))) * Dt / dxc;
-        double CeP = 0.5 * (0.5 * (a2 + fabs(a2)) + 0.5 * (a3 + fabs(a3))) * Dt / dxc;
-        double CeM = 0.5 * (0.5 * (a2 - fabs(a2)) + 0.5 * (a3 - fabs(a3))) * Dt / dxc;
-        double CsP = 0.5 * (0.5 * (vs0 + fabs(vs0)) + 0.5 * (vs1 + fabs(vs1))) * Dt / dyv;
-        double CsM = 0.5 * (0.5 * (vs0 - fabs(vs0)) + 0.5 * (vs1 - fabs(vs1))) * Dt / dyv;
-        double CnP = 0.5 * (0.5 * (vn0 + fabs(vn0)) + 0.5 * (vn1 + fabs(vn1))) * Dt / dyv;
-        double CnM = 0.5 * (0.5 * (vn0 - fabs(vn0)) + 0.5 * (vn1 - fabs(vn1))) * Dt / dyv;
+        double CwP = ddiv(0.5 * (0.5 * (a1 + fabs(a1)) + 0.5 * (a2 + fabs(a2))) * Dt, dxc);
+        double CwM = ddiv(0.5 * (0.5 * (a1 - fabs(a1)) + 0.5 * (a2 - fabs(a2))) * Dt, dxc);
+        double CeP = ddiv(0.5 * (0.5 * (a2 + fabs(a2)) + 0.5 * (a3 + fabs(a3))) * Dt, dxc);
+        double CeM = ddiv(0.5 * (0.5 * (a2 - fabs(a2)) + 0.5 * (a3 - fabs(a3))) * Dt, dxc);
+        double CsP = ddiv(0.5 * (0.5 * (vs0 + fabs(vs0)) + 0.5 * (vs1 + fabs(vs1))) * Dt, dyv);
+        double CsM = ddiv(0.5 * (0.5 * (vs0 - fabs(vs0)) + 0.5 * (vs1 - fabs(vs1))) * Dt, dyv);
+        double CnP = ddiv(0.5 * (0.5 * (vn0 + fabs(vn0)) + 0.5 * (vn1 + fabs(vn1))) * Dt, dyv);
+        double CnM = ddiv(0.5 * (0.5 * (vn0 - fabs(vn0)) + 0.5 * (vn1 - fabs(vn1))) * Dt, dyv);
         double rxPprev = r2rP(a0, a1, a2), rxMprev = r2rM(a1, a2, a3), rxP = r2rP(a1, a2, a3), rxM = r2rM(a2, a3, a4);
         double ryPprev = r2rP(c0, c1, a2), ryMprev = r2rM(c1, a2, c3), ryP = r2rP(c1, a2, c3), ryM = r2rM(a2, c3, c4_);
         double BwP = b2bp<SCH>(a1, a2, rxPprev, CwP), BwM = b2bm<SCH>(a1, a2, rxMprev, CwM);
@@ -281,9 +285,9 @@ __global__ __launch_bounds__(BX *BY) void k_uv(Grid g, Ptrs p, Flags *fl, double
         double BsP = b2bp<SCH>(c1, a2, ryPprev, CsP), BsM = b2bm<SCH>(c1, a2, ryMprev, CsM);
         double BnP = b2bp<SCH>(a2, c3, ryP, CnP), BnM = b2bm<SCH>(a2, c3, ryM, CnM);
         double p1 = CwP * BwP + CwM * BwM - CeP * BeP - CeM * BeM + CsP * BsP + CsM * BsM - CnP * BnP - CnM * BnM;
-        double p2 = Dt * (a2 * (a3 - a1) / (dxe + dxe) + a2 * ((vn0 + vn1) / 2. - (vs0 + vs1) / 2.) / dyv);
+        double p2 = Dt * (ddiv(a2 * (a3 - a1), (dxe + dxe)) + ddiv(a2 * ((vn0 + vn1) / 2. - (vs0 + vs1) / 2.), dyv));
         double adv = p1 + p2;
-        double pres = -TB(g, SW2_T_GE, j) * Dt * (p.eta[IX(g, i + 1, j)] - p.eta[q]) / dxc;  // calc.c:517
+        double pres = ddiv(-TB(g, SW2_T_GE, j) * Dt * (p.eta[IX(g, i + 1, j)] - p.eta[q]), dxc);  // calc.c:517
         du = adv + pres;                                                                      // core.c:224
     }
     {  // BODYV, calc.c:205-237
@@ -292,14 +296,14 @@ __global__ __launch_bounds__(BX *BY) void k_uv(Grid g, Ptrs p, Flags *fl, double
         double uw_m1j1 = UW(i - 1, j + 1), uw_m1j = UW(i - 1, j), uw_j1 = UW(i, j + 1), uw_j = UW(i, j);
         double a0 = VW(i - 2, j), a1 = VW(i - 1, j), a2 = VW(i, j), a3 = VW(i + 1, j), a4 = VW(i + 2, j);
         double c0 = VW(i, j - 2), c1 = VW(i, j - 1), c3 = VW(i, j + 1), c4_ = VW(i, j + 2);
-        double CwP = 0.5 * (0.5 * (uw_m1j1 + fabs(uw_m1j1)) + 0.5 * (uw_m1j + fabs(uw_m1j))) * Dt / dxv;
-        double CwM = 0.5 * (0.5 * (uw_m1j1 - fabs(uw_m1j1)) + 0.5 * (uw_m1j - fabs(uw_m1j))) * Dt / dxv;
-        double CeP = 0.5 * (0.5 * (uw_j1 + fabs(uw_j1)) + 0.5 * (uw_j + fabs(uw_j))) * Dt / dxv;
-        double CeM = 0.5 * (0.5 * (uw_j1 - fabs(uw_j1)) + 0.5 * (uw_j - fabs(uw_j))) * Dt / dxv;
-        double CsP = 0.5 * (0.5 * (c1 + fabs(c1)) + 0.5 * (a2 + fabs(a2))) * Dt / dyc;
-        double CsM = 0.5 * (0.5 * (c1 - fabs(c1)) + 0.5 * (a2 - fabs(a2))) * Dt / dyc;
-        double CnP = 0.5 * (0.5 * (a2 + fabs(a2)) + 0.5 * (c3 + fabs(c3))) * Dt / dyc;
-        double CnM = 0.5 * (0.5 * (a2 - fabs(a2)) + 0.5 * (c3 - fabs(c3))) * Dt / dyc;
+        double CwP = ddiv(0.5 * (0.5 * (uw_m1j1 + fabs(uw_m1j1)) + 0.5 * (uw_m1j + fabs(uw_m1j))) * Dt, dxv);
+        double CwM = ddiv(0.5 * (0.5 * (uw_m1j1 - fabs(uw_m1j1)) + 0.5 * (uw_m1j - fabs(uw_m1j))) * Dt, dxv);
+        double CeP = ddiv(0.5 * (0.5 * (uw_j1 + fabs(uw_j1)) + 0.5 * (uw_j + fabs(uw_j))) * Dt, dxv);
+        double CeM = ddiv(0.5 * (0.5 * (uw_j1 - fabs(uw_j1)) + 0.5 * (uw_j - fabs(uw_j))) * Dt, dxv);
+        double CsP = ddiv(0.5 * (0.5 * (c1 + fabs(c1)) + 0.5 * (a2 + fabs(a2))) * Dt, dyc);
+        double CsM = ddiv(0.5 * (0.5 * (c1 - fabs(c1)) + 0.5 * (a2 - fabs(a2))) * Dt, dyc);
+        double CnP = ddiv(0.5 * (0.5 * (a2 + fabs(a2)) + 0.5 * (c3 + fabs(c3))) * Dt, dyc);
+        double CnM = ddiv(0.5 * (0.5 * (a2 - fabs(a2)) + 0.5 * (c3 - fabs(c3))) * Dt, dyc);
         double rxPprev = r2rP(a0, a1, a2), rxMprev = r2rM(a1, a2, a3), rxP = r2rP(a1, a2, a3), rxM = r2rM(a2, a3, a4);
         double ryPprev = r2rP(c0, c1, a2), ryMprev = r2rM(c1, a2, c3), ryP = r2rP(c1, a2, c3), ryM = r2rM(a2, c3, c4_);
         double BwP = b2bp<SCH>(a1, a2, rxPprev, CwP), BwM = b2bm<SCH>(a1, a2, rxMprev, CwM);
@@ -307,9 +311,9 @@ __global__ __launch_bounds__(BX *BY) void k_uv(Grid g, Ptrs p, Flags *fl, double
         double BsP = b2bp<SCH>(c1, a2, ryPprev, CsP), BsM = b2bm<SCH>(c1, a2, ryMprev, CsM);
         double BnP = b2bp<SCH>(a2, c3, ryP, CnP), BnM = b2bm<SCH>(a2, c3, ryM, CnM);
         double p1 = CwP * BwP + CwM * BwM - CeP * BeP - CeM * BeM + CsP * BsP + CsM * BsM - CnP * BnP - CnM * BnM;
-        double p2 = Dt * (a2 * ((uw_j + uw_j1) / 2. - (uw_m1j + uw_m1j1) / 2.) / dxv + a2 * (c3 - c1) / dyn2);
+        double p2 = Dt * (ddiv(a2 * ((uw_j + uw_j1) / 2. - (uw_m1j + uw_m1j1) / 2.), dxv) + ddiv(a2 * (c3 - c1), dyn2));
         double adv = p1 + p2;
-        double pres = -TB(g, SW2_T_GN, j) * Dt * (p.eta[IX(g, i, j + 1)] - p.eta[q]) / TB(g, SW2_T_DYC, j + 1);  // calc.c:538
+        double pres = ddiv(-TB(g, SW2_T_GN, j) * Dt * (p.eta[IX(g, i, j + 1)] - p.eta[q]), TB(g, SW2_T_DYC, j + 1));  // calc.c:538
         dv = adv + pres;
     }
     du -= p.u[q] * sigma * Dt;  // calc.c:396-397
@@ -334,9 +338,9 @@ __global__ __launch_bounds__(BX *BY) void k_cor_u(Grid g, Ptrs p, Flags *fl, con
     const int q = IX(g, i, j);
     double semiU = 0.25 * (p.v[q] + p.v[IX(g, i + 1, j)] + p.v[IX(g, i, j - 1)] + p.v[IX(g, i + 1, j - 1)]);
     double U = p.u[q];
-    double Alpha = Dt * TB(g, SW2_T_FE, j) + (Dt * U * TB(g, SW2_T_SINC, j)) / TB(g, SW2_T_RZC, j);
+    double Alpha = Dt * TB(g, SW2_T_FE, j) + ddiv((Dt * U * TB(g, SW2_T_SINC, j)), TB(g, SW2_T_RZC, j));
     double Beta = 0.25 * Alpha * Alpha;
-    p.u[q] = (U + duab[q] - Beta * U + Alpha * semiU) / (1.0 + Beta);
+    p.u[q] = ddiv((U + duab[q] - Beta * U + Alpha * semiU), (1.0 + Beta));
 }
 
 __global__ __launch_bounds__(BX *BY) void k_cor_v(Grid g, Ptrs p, Flags *fl, const double *dvab, double Dt) {
@@ -345,9 +349,9 @@ __global__ __launch_bounds__(BX *BY) void k_cor_v(Grid g, Ptrs p, Flags *fl, con
     // U interior already holds the new values, its halo the previous ones
     double semiV = 0.25 * (p.u[IX(g, i - 1, j + 1)] + p.u[IX(g, i, j + 1)] + p.u[q] + p.u[IX(g, i - 1, j)]);
     double V = p.v[q];
-    double Alpha = Dt * TB(g, SW2_T_FN, j) + (Dt * semiV * TB(g, SW2_T_SINN, j)) / TB(g, SW2_T_RZN, j);
+    double Alpha = Dt * TB(g, SW2_T_FN, j) + ddiv((Dt * semiV * TB(g, SW2_T_SINN, j)), TB(g, SW2_T_RZN, j));
     double Beta = 0.25 * Alpha * Alpha;
-    p.v[q] = (V + dvab[q] - Beta * V - Alpha * semiV) / (1.0 + Beta);
+    p.v[q] = ddiv((V + dvab[q] - Beta * V - Alpha * semiV), (1.0 + Beta));
 }
 
 // ------------------------------------------------ Laplacians (laplacian.c:51-151)
@@ -355,18 +359,18 @@ __global__ __launch_bounds__(BX *BY) void k_lapv_zd(Grid g, Flags *fl, const dou
                                                       double *di) {
     CELL();
     const int q = IX(g, i, j);
-    double a = (1.0 / TB(g, SW2_T_DXV, j)), b = -(1.0 / TB(g, SW2_T_DYC, j + 1));
+    double a = TB(g, SW2_T_IDXV, j), b = -(TB(g, SW2_T_IDYC, j + 1));
     ze[q] = a * ((vin[IX(g, i + 1, j)] - vin[q]) +
                  b * (TB(g, SW2_T_DXC, j + 1) * uin[IX(g, i, j + 1)] - TB(g, SW2_T_DXC, j) * uin[q]));
-    a = (1.0 / TB(g, SW2_T_DXE, j));
-    b = (1.0 / TB(g, SW2_T_DYN, j));
+    a = TB(g, SW2_T_IDXE, j);
+    b = TB(g, SW2_T_IDYN, j);
     di[q] = a * ((uin[q] - uin[IX(g, i - 1, j)]) + b * (TB(g, SW2_T_DXV, j) * vin[q] - TB(g, SW2_T_DXV, j - 1) * vin[IX(g, i, j - 1)]));
 }
 
 __global__ void k_lapv_fix1(Grid g, Flags *fl, const double *uin, double *ze, double *di) {
     const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
     if (fl->fail_step || i > g.nx) return;
-    di[IX(g, i, g.ny + 1)] = (uin[IX(g, i, g.ny)] - uin[IX(g, i - 1, g.ny)]) / TB(g, SW2_T_DXE, g.ny);
+    di[IX(g, i, g.ny + 1)] = ddiv((uin[IX(g, i, g.ny)] - uin[IX(g, i - 1, g.ny)]), TB(g, SW2_T_DXE, g.ny));
     ze[IX(g, i, 0)] = 0;
 }
 
@@ -374,10 +378,10 @@ __global__ __launch_bounds__(BX *BY) void k_lapv_out(Grid g, Flags *fl, const do
                                                        double *vo) {
     CELL();
     const int q = IX(g, i, j);
-    double a = -(1.0 / TB(g, SW2_T_DYN, j)), b = (1.0 / TB(g, SW2_T_DXC, j));
+    double a = -(TB(g, SW2_T_IDYN, j)), b = TB(g, SW2_T_IDXC, j);
     uo[q] = a * (ze[q] - ze[IX(g, i, j - 1)]) + b * (di[IX(g, i + 1, j)] - di[q]);
-    a = (1.0 / TB(g, SW2_T_DXV, j));
-    b = (1.0 / TB(g, SW2_T_DYC, j + 1));
+    a = TB(g, SW2_T_IDXV, j);
+    b = TB(g, SW2_T_IDYC, j + 1);
     vo[q] = a * (ze[q] - ze[IX(g, i - 1, j)]) + b * (di[IX(g, i, j + 1)] - di[q]);
 }
 
@@ -393,16 +397,16 @@ __global__ void k_lapv_fix2(Grid g, Flags *fl, double *uo, double *vo) {
 __global__ __launch_bounds__(BX *BY) void k_lape_px(Grid g, Flags *fl, const double *ein, double *psi_, double *xi) {
     CELL();
     const int q = IX(g, i, j);
-    double fp = TB(g, SW2_T_RMC, j) / TB(g, SW2_T_RZC, j), fx = TB(g, SW2_T_RZN, j) / TB(g, SW2_T_RMN, j);
-    psi_[q] = fp * (ein[IX(g, i + 1, j)] - ein[q]) / (TB(g, SW2_T_DXC, j));
-    xi[q] = fx * (ein[IX(g, i, j + 1)] - ein[q]) / (TB(g, SW2_T_DYC, j));
+    double fp = TB(g, SW2_T_FP, j), fx = TB(g, SW2_T_FX, j);
+    psi_[q] = ddiv(fp * (ein[IX(g, i + 1, j)] - ein[q]), (TB(g, SW2_T_DXC, j)));
+    xi[q] = ddiv(fx * (ein[IX(g, i, j + 1)] - ein[q]), (TB(g, SW2_T_DYC, j)));
 }
 
 __global__ __launch_bounds__(BX *BY) void k_lape_out(Grid g, Flags *fl, const double *psi_, const double *xi, double *eo) {
     CELL();
     const int q = IX(g, i, j);
-    double kk = 1.0 / (TB(g, SW2_T_RMC, j) * TB(g, SW2_T_RZC, j));
-    eo[q] = kk * ((psi_[q] - psi_[IX(g, i - 1, j)]) / (TB(g, SW2_T_DXE, j)) + (xi[q] - xi[IX(g, i, j - 1)]) / (TB(g, SW2_T_DYN, j)));
+    double kk = TB(g, SW2_T_KK, j);
+    eo[q] = kk * (ddiv((psi_[q] - psi_[IX(g, i - 1, j)]), (TB(g, SW2_T_DXE, j))) + ddiv((xi[q] - xi[IX(g, i, j - 1)]), (TB(g, SW2_T_DYN, j))));
 }
 
 __global__ void k_lape_fix(Grid g, Flags *fl, double *eo) {
@@ -592,7 +596,7 @@ extern "C" int sw2_session_create(sw *SW, int form, sw2_session **out) {
     int rc = 0;
     const int R = s->g.R, NTD = SW2_NT + 2;
     double *tab = (double *)calloc((size_t)NTD * R, sizeof(double));
-    sw2_row_tables(SW, tab);
+    const int tables_ok = sw2_row_tables(SW, tab);
     // windx as the fused kernels read it: the row value everywhere except in
     // the four corner blocks, which keep the y-wrap of the first halo update
     int wind_rows_ok = 1;
@@ -611,7 +615,7 @@ extern "C" int sw2_session_create(sw *SW, int form, sw2_session **out) {
         memcpy(&b, &SW->windy[q], sizeof b);
         if (b) windy_zero = 0;
     }
-    s->fused = !force_phased && wind_rows_ok && windy_zero && !s->tracer_active && SW->num_Gaussians == 0 &&
+    s->fused = !force_phased && tables_ok && wind_rows_ok && windy_zero && !s->tracer_active && SW->num_Gaussians == 0 &&
                SW->num_vortices == 0 && SW->nu6 == 0 && SW->nx >= 2 * FTX && SW->ny >= 4 &&
                !(SW->nu2 != 0 && SW->nu4 == 0);
     s->hvmode = s->hv ? 2 : 0;
@@ -788,11 +792,13 @@ template <int SCH, int HV>
 static void launch_fused(sw2_session *s, int n, int k, int slot) {
     dim3 G((s->g.nx + FTX - 1) / FTX, (s->g.ny + FTY - 1) / FTY);
     double sig2 = 2.0 * s->Dt * (s->sigma);
-    kf_eta<SCH, HV><<<G, FNT, 0, s->st>>>(s->g, s->fl, s->eta, s->u, s->v, s->eta2, s->Deta[0], s->Deta[1], s->Deta[2], n, k,
-                                           slot, s->Dt, s->depth, sig2, s->c2, s->c4, s->c6, s->courant, s->rest_x,
-                                           s->rest_y);
+    double tsig2 = 2.0 * s->Dt * (s->tracer_dis);
+    kf_eta<SCH, HV><<<G, FNT, 0, s->st>>>(s->g, s->fl, s->eta, s->u, s->v, s->eta2, s->Deta[0], s->Deta[1], s->Deta[2], s->tr,
+                                           s->Dtr[0], n, k, slot, s->Dt, s->depth, sig2, tsig2, s->c2, s->c4, s->courant,
+                                           s->rest_x, s->rest_y);
     kf_uv<SCH, HV><<<G, FNT, 0, s->st>>>(s->g, s->fl, s->eta2, s->u, s->v, s->u2, s->v2, s->Du[0], s->Du[1], s->Du[2], s->Dv[0],
-                                          s->Dv[1], s->Dv[2], n, k, slot ^ 1, s->Dt, s->sigma, s->c2, s->c4);
+                                          s->Dv[1], s->Dv[2], s->tr, s->Dtr[0], s->Dtr[1], s->Dtr[2], n, k, slot ^ 1, s->Dt,
+                                          s->sigma, s->c2, s->c4);
 }
 
 template <int SCH>
@@ -819,13 +825,8 @@ static int step_fused(sw2_session *s) {
         k_cfl<<<G, B, 0, s->st>>>(s->g, p, s->fl, slot);
         s->cfl_ready = 1;
     }
-    {  // tracer: runs only once u or v went non-finite (it is identically zero)
-        double sig2 = 2.0 * s->Dt * (s->tracer_dis);
-        SCH_SWITCH(launch_flux, s, p, true, s->tr, sig2, s->Dtr[0]);
-        k_tracer_update<<<G, B, 0, s->st>>>(s->g, s->fl, s->tr, s->Dtr[0], s->Dtr[1], s->Dtr[2], k, 1);
-        rot(s->Dtr);
-        launch_halo(s, s->tr, 1);
-    }
+    // two kernels; the tracer (identically zero until u or v goes
+    // non-finite) is advanced inside them once that happens
     SCH_SWITCH(launch_fused_hv, s, n, k, slot);
     swap_ptr(&s->eta, &s->eta2);
     swap_ptr(&s->u, &s->u2);
@@ -833,11 +834,11 @@ static int step_fused(sw2_session *s) {
     rot(s->Deta);
     rot(s->Du);
     rot(s->Dv);
+    rot(s->Dtr);
     if (s->perturbs_dirty) {
         k_zero<<<(s->S + 255) / 256, 256, 0, s->st>>>(s->S, s->pert);
         s->perturbs_dirty = 0;
     }
-    k_check<<<G, B, 0, s->st>>>(s->g, s->fl, s->tr, n, 2, 1);
     s->n = n;
     s->t = t;
     return 0;
@@ -980,20 +981,20 @@ extern "C" int sw2_session_state(sw2_session *s, int *n, double *t) {
 
 extern "C" void sw2_session_destroy(sw2_session *s) {
     if (!s) return;
-    if (s->st) hipStreamSynchronize(s->st);
+    if (s->st) (void)hipStreamSynchronize(s->st);
     double *all[] = {s->d_tab, s->eta, s->u, s->v, s->eta2, s->u2, s->v2, s->tr, s->pert, s->wx, s->wy, s->duab, s->dvab, s->Lu, s->Lv, s->LLu, s->LLv,
                      s->LLLu, s->LLLv, s->ze, s->di, s->le, s->lle, s->llle, s->psi, s->xi, s->pinc, s->ptr, s->pdu, s->pdv};
     for (auto p : all)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     for (int k = 0; k < 3; k++) {
-        if (s->Deta[k]) hipFree(s->Deta[k]);
-        if (s->Du[k]) hipFree(s->Du[k]);
-        if (s->Dv[k]) hipFree(s->Dv[k]);
-        if (s->Dtr[k]) hipFree(s->Dtr[k]);
+        if (s->Deta[k]) (void)hipFree(s->Deta[k]);
+        if (s->Du[k]) (void)hipFree(s->Du[k]);
+        if (s->Dv[k]) (void)hipFree(s->Dv[k]);
+        if (s->Dtr[k]) (void)hipFree(s->Dtr[k]);
     }
-    if (s->pmask) hipFree(s->pmask);
-    if (s->fl) hipFree(s->fl);
-    if (s->st) hipStreamDestroy(s->st);
+    if (s->pmask) (void)hipFree(s->pmask);
+    if (s->fl) (void)hipFree(s->fl);
+    if (s->st) (void)hipStreamDestroy(s->st);
     for (int k = 0; k < 8; k++) {
         free(s->h_inc[k]);
         free(s->h_trv[k]);

@@ -250,7 +250,7 @@ static void coords(sw *SW) {
     }
 }
 
-void sw2_row_tables(sw *SW, double *tab) {
+int sw2_row_tables(sw *SW, double *tab) {
     int R = SW->ny + 4;
     map *M = SW->M;
     for (int j = -1; j <= SW->ny + 2; j++) {
@@ -274,6 +274,33 @@ void sw2_row_tables(sw *SW, double *tab) {
         tab[SW2_T_RMC * R + r] = g_rM(SW, latc);
         tab[SW2_T_RMN * R + r] = g_rM(SW, latn);
     }
+    static const int inv[][2] = {{SW2_T_IDXE, SW2_T_DXE}, {SW2_T_IDXC, SW2_T_DXC}, {SW2_T_IDXV, SW2_T_DXV},
+                                 {SW2_T_IDYN, SW2_T_DYN}, {SW2_T_IDYC, SW2_T_DYC}, {SW2_T_IDYV, SW2_T_DYV},
+                                 {SW2_T_IRZC, SW2_T_RZC}, {SW2_T_IRZN, SW2_T_RZN}};
+    for (int r = 0; r < R; r++) {
+        for (unsigned k = 0; k < sizeof inv / sizeof *inv; k++) tab[inv[k][0] * R + r] = 1.0 / tab[inv[k][1] * R + r];
+        double dxe = tab[SW2_T_DXE * R + r];
+        tab[SW2_T_DXE2 * R + r] = dxe + dxe;
+        tab[SW2_T_IDXE2 * R + r] = 1.0 / (dxe + dxe);
+        double dyn2 = (r + 1 < R) ? tab[SW2_T_DYN * R + r] + tab[SW2_T_DYN * R + r + 1] : 1.0;
+        tab[SW2_T_DYN2 * R + r] = dyn2;
+        tab[SW2_T_IDYN2 * R + r] = 1.0 / dyn2;
+        double rmc = tab[SW2_T_RMC * R + r], rzc = tab[SW2_T_RZC * R + r];
+        tab[SW2_T_FP * R + r] = rmc / rzc;
+        tab[SW2_T_FX * R + r] = tab[SW2_T_RZN * R + r] / tab[SW2_T_RMN * R + r];
+        tab[SW2_T_KK * R + r] = 1.0 / (rmc * rzc);
+        tab[SW2_T_GF * R + r] = SW->Dt * tab[SW2_T_SINC * R + r] / rzc;
+    }
+    /* divisors taken through their reciprocals: positive, in [2^-60, 2^60] */
+    static const int dv[] = {SW2_T_DXE, SW2_T_DXC, SW2_T_DXV, SW2_T_DYN, SW2_T_DYC, SW2_T_DYV,
+                             SW2_T_RZC, SW2_T_RZN, SW2_T_DXE2, SW2_T_DYN2};
+    int ok = 1;
+    for (int r = 1; r <= SW->ny + 2; r++) /* rows j = 0 .. ny+1 */
+        for (unsigned k = 0; k < sizeof dv / sizeof *dv; k++) {
+            double b = tab[dv[k] * R + r];
+            if (!(b >= 0x1p-60 && b <= 0x1p+60)) ok = 0;
+        }
+    return ok;
 }
 
 void sw2_cfl_rest(sw *SW, double *mx, double *my) {

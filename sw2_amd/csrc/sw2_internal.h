@@ -22,10 +22,24 @@ enum {
     SW2_T_DXE = 0, SW2_T_DXC, SW2_T_DXV, SW2_T_DYN, SW2_T_DYC, SW2_T_DYV,
     SW2_T_GE, SW2_T_GN, SW2_T_GC, SW2_T_FE, SW2_T_FN,
     SW2_T_SINC, SW2_T_RZC, SW2_T_SINN, SW2_T_RZN, SW2_T_RMC, SW2_T_RMN,
+    /* derived on the host, each with SW2's own expression and association:
+     * reciprocals RN(1/x) of the divisors (the GPU divides through them,
+     * exactly), 1.0/x factors SW2 writes as such (laplacian.c:58-89), the
+     * row-constant quotients of laplacian_eta and do_eta_geofactor */
+    SW2_T_IDXE, SW2_T_IDXC, SW2_T_IDXV, SW2_T_IDYN, SW2_T_IDYC, SW2_T_IDYV,
+    SW2_T_IRZC, SW2_T_IRZN,
+    SW2_T_DXE2, SW2_T_IDXE2, /* Dxe+Dxe (adv_Pu2, calc.c:198) and RN(1/.) */
+    SW2_T_DYN2, SW2_T_IDYN2, /* Dyn(j)+Dyn(j+1) (adv_Pv2, calc.c:235) and RN(1/.) */
+    SW2_T_FP,                /* rM(latc)/rZ(latc) (laplacian.c:121) */
+    SW2_T_FX,                /* rZ(latn)/rM(latn) (laplacian.c:122) */
+    SW2_T_KK,                /* 1.0/(rM(latc)*rZ(latc)) (laplacian.c:126) */
+    SW2_T_GF,                /* (Dt*sin(latc))/rZ(latc) (calc.c:420) */
     SW2_NT
 };
-/* Fill tab[SW2_NT*(ny+4)] from the world. */
-void sw2_row_tables(sw *SW, double *tab);
+/* Fill tab[SW2_NT*(ny+4)] from the world.  Returns 1 when every divisor the
+ * GPU divides through by reciprocal (rows 0 .. ny+1) lies in the range that
+ * division accepts, 0 otherwise. */
+int sw2_row_tables(sw *SW, double *tab);
 
 /* Gravity-wave fallback of compute_Dt (calc.c:721-729): the two maxima over
  * the interior when the flow is at rest. */
