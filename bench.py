@@ -1,14 +1,26 @@
 """Benchmark of the GPU time step on the 4096x2048 FP64 workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps C]
+    python bench.py --copy-calib        (a device copy of known size, for
+                                         scaling the profiler's byte counters)
 
 A step is one pass of SW2's per-step update (core.c:134-338) over the
 4096x2048 channel, lat -85..85, Dt 10 s, del-4 5.97e10 m^4/s, minmod, jet
 as background wind plus one storm bump; the fields stay resident in HBM.
-Prints one JSON line: whole-job cell-updates/s, the HBM roofline of the
-step (120 algorithmic bytes per cell-update) from HIP events on the
-session's stream, and the oracle (the CPU restatement of SW2's path) timed
-on a bounded sample of the same workload on this host.
+
+Prints one JSON line:
+  value         whole-job cell-updates/s over the K timed steps (wall clock
+                between synchronisations);
+  roofline      the dominant kernel, the momentum kernel of the fused step:
+                88 algorithmic bytes per cell (reads eta, u, v and four
+                Adams-Bashforth histories, writes u, v and two tendencies)
+                over its average launch time from HIP events around each
+                launch on the session's stream; the whole step at 120
+                algorithmic bytes per cell-update beside it; `traffic` is the
+                measured HBM bytes per launch from the profiler summary in
+                profiles/ (byte counters scaled by a device copy), if present;
+  cpu_baseline  the oracle (the CPU restatement of SW2's path) timed on a
+                bounded sample of the same workload on this host.
 """
 import argparse
 import json
@@ -19,8 +31,11 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
-BYTES_PER_CELL = {"f64": 120}  # eta,u,v read+write (48) + 6 AB histories read (48) + 3 tendencies written (24)
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+STEP_BYTES = 120       # per cell-update: eta,u,v read+written (48) + 6 histories read (48) + 3 tendencies written (24)
+UV_BYTES = 88          # momentum kernel per cell: eta,u,v + 4 histories read (56), u,v + 2 tendencies written (32)
+ETA_BYTES = 56         # height kernel per cell: eta,u,v + 2 histories read (40), eta + 1 tendency written (16)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_4096x2048_fp64.json")
 
 
 def parse():
@@ -29,8 +44,32 @@ def parse():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample (0: skip)")
+    ap.add_argument("--kernel-steps", type=int, default=100, help="steps timed per kernel after the timed region")
     ap.add_argument("--workload", default="c1", choices=["c0", "c1"])
+    ap.add_argument("--copy-calib", action="store_true", help="only copy 3 x 512 MiB device to device and exit")
     return ap.parse_args()
+
+
+def copy_calib():
+    """Three device-to-device copies of 512 MiB through the HIP runtime: the
+    reference transfer the profiler's FETCH_SIZE / WRITE_SIZE are scaled by."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = 512 * 1024 * 1024
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(a, 1, ctypes.c_size_t(n)) != 0:
+        raise RuntimeError("hipMemset failed")
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    if hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipDeviceSynchronize failed")
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calib_bytes": 3 * n}), flush=True)
 
 
 def cpu_baseline(p_fn, bumps, steps):
@@ -50,15 +89,26 @@ def cpu_baseline(p_fn, bumps, steps):
             "sample": f"{p.nx}x{p.ny}, {steps} steps of the same workload, single thread, {dt:.1f} s"}
 
 
+def measured_traffic():
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus})", file=sys.stderr)
         sys.exit(2)
+    if a.copy_calib:
+        copy_calib()
+        return
     from sw2_amd import Session, cases
 
     wl = a.workload
-    total = a.warmup + a.steps
+    total = a.warmup + a.steps + a.kernel_steps
     p_fn = lambda n: cases.config_params(wl, steps=n)
     bumps = cases.single_bump(2024)
     p = p_fn(total)
@@ -67,39 +117,49 @@ def main():
     s.begin(0, p.t0)
     if a.warmup:
         s.step(a.warmup)
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:
-        sync = lambda: None
-    sync()
+        s.timed_steps(0)  # drains the stream
     t0 = time.perf_counter()
-    dev_ms = s.timed_steps(a.steps)
-    sync()
+    dev_ms = s.timed_steps(a.steps)  # returns once the last step has completed
     wall = time.perf_counter() - t0
+    kt = s.kernel_times(a.kernel_steps) if (a.kernel_steps > 0 and s.fused) else None
     failed, msg = s.failed_step()
     if failed:
         print(f"step {failed} failed SW2's guard: {msg}", file=sys.stderr)
         sys.exit(1)
     cells = p.nx * p.ny
     value = cells * a.steps / wall
-    step_s = dev_ms / 1e3 / a.steps
-    achieved = BYTES_PER_CELL["f64"] * cells / step_s / 1e9
+    step_ms = dev_ms / a.steps
+    step_gbs = STEP_BYTES * cells / (step_ms / 1e3) / 1e9
+    fused = s.fused
     s.close()
     w.close()
     cpu = cpu_baseline(p_fn, bumps, a.cpu_steps) if a.cpu_steps > 0 else None
     names = {"c1": "4096x2048 FP64 jet + storm, lat -85..85, Dt 10 s, del-4 5.97e10, minmod",
              "c0": "360x180 FP64 jet + storm, lat -80..80, Dt 60 s, del-4 1e15, minmod"}
+    if kt:
+        eta_ms, uv_ms = kt[0] / a.kernel_steps, kt[1] / a.kernel_steps
+        uv_gbs = UV_BYTES * cells / (uv_ms / 1e3) / 1e9
+        tr = measured_traffic() if wl == "c1" else None
+        roof = {"bound": "hbm", "achieved": uv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": uv_gbs / HBM_PEAK_GBS,
+                "traffic": tr["kf_uv"]["bytes"] if tr else None,
+                "kernel": "kf_uv (momentum kernel of the fused step)", "bytes_per_cell": UV_BYTES,
+                "kernel_ms": uv_ms, "eta_kernel_ms": eta_ms,
+                "eta_kernel_gbs": ETA_BYTES * cells / (eta_ms / 1e3) / 1e9,
+                "step": {"bytes_per_cell": STEP_BYTES, "device_ms": step_ms, "achieved": step_gbs,
+                         "frac": step_gbs / HBM_PEAK_GBS}}
+        if tr:
+            roof["traffic_source"] = tr.get("source")
+    else:
+        roof = {"bound": "hbm", "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": step_gbs / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "whole step (one kernel per phase)", "bytes_per_cell": STEP_BYTES, "device_ms": step_ms}
     out = {
         "metric": "cell-updates/s", "value": value, "unit": "cell-updates/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall * 1e3 / a.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (balanced background jet + seeded storm bump)",
-        "config": {"workload": names[wl], "grid": [p.nx, p.ny], "cells": cells},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "whole step (all kernels of one step, HIP events on the session stream)",
-                     "device_ms_per_step": dev_ms / a.steps},
+        "config": {"workload": names[wl], "grid": [p.nx, p.ny], "cells": cells, "path": "fused" if fused else "phased"},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

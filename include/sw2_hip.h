@@ -233,6 +233,11 @@ int sw2_session_step(sw2_session *s, int nsteps);
 /* Advance nsteps steps and return their device time in ms, measured with
  * HIP events on the session's stream. */
 int sw2_session_time(sw2_session *s, int nsteps, double *ms);
+/* Advance nsteps steps on the fused path and return the summed device time
+ * of each of its two kernels, ms[0] the height kernel and ms[1] the momentum
+ * kernel (HIP events around every launch on the session's stream; nk >= 2).
+ * Fails on a session that steps one kernel per phase. */
+int sw2_session_kernel_times(sw2_session *s, int nsteps, double *ms, int nk);
 /* Copy eta, u, v, tracer, perturbs (halos included) back into SW's arrays. */
 int sw2_session_download(sw2_session *s, sw *SW);
 /* First step whose CFL or stats guard failed (0 if none) and its message. */

@@ -163,6 +163,13 @@ class Session:
         _check(lib().sw2_session_time(self.h, n, byref(ms)), "sw2_session_time")
         return ms.value
 
+    def kernel_times(self, n):
+        """Advance n steps on the fused path; summed device ms of the height
+        kernel and of the momentum kernel (HIP events around each launch)."""
+        ms = (c_double * 2)()
+        _check(lib().sw2_session_kernel_times(self.h, n, ms, 2), "sw2_session_kernel_times")
+        return ms[0], ms[1]
+
     def download(self):
         _check(lib().sw2_session_download(self.h, byref(self.world.sw)), "sw2_session_download")
 

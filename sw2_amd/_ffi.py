@@ -80,7 +80,8 @@ EXPORTS = [
     "sw2_init_params", "sw2_read_case", "sw2_free_params", "sw2_eval_expr",
     "halo_update", "do_channel", "calc_energy_balance", "sw2_mass",
     "save_binary_sw", "load_binary_sw",
-    "sw2_session_create", "sw2_session_begin", "sw2_session_step", "sw2_session_time", "sw2_session_download",
+    "sw2_session_create", "sw2_session_begin", "sw2_session_step", "sw2_session_time", "sw2_session_kernel_times",
+    "sw2_session_download",
     "sw2_session_failed_step", "sw2_session_last_cfl", "sw2_session_fused", "sw2_session_state", "sw2_session_destroy",
     "sw2_last_error", "sw2_build_info",
 ]
@@ -114,6 +115,7 @@ def lib():
     L.sw2_session_begin.argtypes = [c_void_p, c_int, c_double]
     L.sw2_session_step.argtypes = [c_void_p, c_int]
     L.sw2_session_time.argtypes = [c_void_p, c_int, P]
+    L.sw2_session_kernel_times.argtypes = [c_void_p, c_int, P, c_int]
     L.sw2_session_download.argtypes = [c_void_p, POINTER(SW)]
     L.sw2_session_failed_step.argtypes = [c_void_p, ctypes.c_char_p, c_int]
     L.sw2_session_last_cfl.argtypes = [c_void_p, P]

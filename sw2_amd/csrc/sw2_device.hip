@@ -516,6 +516,8 @@ struct sw2_session {
     int hvmode;     // fused path: 0 none, 2 del-2 and del-4 (del-6 runs phased)
     int cfl_ready;  // CFL maxima of the next step already reduced
     double *eta2, *u2, *v2;  // ping-pong partners of eta, u, v on the fused path
+    hipEvent_t *kev;         // per-kernel timing (sw2_session_kernel_times): 3 events a step
+    int kev_i;
     double t, Dt, depth, sigma, tracer_dis, courant, rest_x, rest_y;
     double c2, c4, c6;
     double *d_tab;
@@ -791,14 +793,16 @@ static int host_perturbations(sw2_session *s, int n, double t, int *eta_changed)
 template <int SCH, int HV>
 static void launch_fused(sw2_session *s, int n, int k, int slot) {
     dim3 G((s->g.nx + FTX - 1) / FTX, (s->g.ny + FTY - 1) / FTY);
-    double sig2 = 2.0 * s->Dt * (s->sigma);
-    double tsig2 = 2.0 * s->Dt * (s->tracer_dis);
-    kf_eta<SCH, HV><<<G, FNT, 0, s->st>>>(s->g, s->fl, s->eta, s->u, s->v, s->eta2, s->Deta[0], s->Deta[1], s->Deta[2], s->tr,
-                                           s->Dtr[0], n, k, slot, s->Dt, s->depth, sig2, tsig2, s->c2, s->c4, s->courant,
-                                           s->rest_x, s->rest_y);
-    kf_uv<SCH, HV><<<G, FNT, 0, s->st>>>(s->g, s->fl, s->eta2, s->u, s->v, s->u2, s->v2, s->Du[0], s->Du[1], s->Du[2], s->Dv[0],
-                                          s->Dv[1], s->Dv[2], s->tr, s->Dtr[0], s->Dtr[1], s->Dtr[2], n, k, slot ^ 1, s->Dt,
-                                          s->sigma, s->c2, s->c4);
+    EtaArgs ea{s->eta, s->u, s->v, s->Deta[1], s->Deta[2], s->tr, s->eta2, s->Deta[0], s->Dtr[0], n, k, slot,
+               s->Dt, s->depth, 2.0 * s->Dt * (s->sigma), 2.0 * s->Dt * (s->tracer_dis), s->c2, s->c4, s->courant,
+               s->rest_x, s->rest_y};
+    if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i], s->st);
+    kf_eta<SCH, HV><<<G, NTE, 0, s->st>>>(s->g, s->fl, ea);
+    if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 1], s->st);
+    UVArgs ua{s->eta2, s->u, s->v, s->Du[1], s->Du[2], s->Dv[1], s->Dv[2], s->Dtr[0], s->Dtr[1], s->Dtr[2],
+              s->u2, s->v2, s->Du[0], s->Dv[0], s->tr, n, k, slot ^ 1, s->Dt, s->sigma, s->c2, s->c4};
+    kf_uv<SCH, HV><<<G, NTU, 0, s->st>>>(s->g, s->fl, ua);
+    if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 2], s->st);
 }
 
 template <int SCH>
@@ -933,6 +937,47 @@ extern "C" int sw2_session_time(sw2_session *s, int nsteps, double *ms) {
     *ms = f;
     HIPCHECK(hipEventDestroy(e0));
     HIPCHECK(hipEventDestroy(e1));
+    return 0;
+}
+
+extern "C" int sw2_session_kernel_times(sw2_session *s, int nsteps, double *ms, int nk) {
+    if (!s->fused) {
+        sw2_set_error("per-kernel times are measured on the fused path only");
+        return -1;
+    }
+    if (nk < 2 || nsteps < 1) {
+        sw2_set_error("need nk >= 2 and nsteps >= 1");
+        return -1;
+    }
+    hipEvent_t *ev = (hipEvent_t *)calloc((size_t)3 * nsteps, sizeof(hipEvent_t));
+    int rc = 0;
+    for (int q = 0; q < 3 * nsteps && !rc; q++)
+        if (hipEventCreate(&ev[q]) != hipSuccess) rc = -1;
+    s->kev = ev;
+    for (int q = 0; q < nsteps && !rc; q++) {
+        s->kev_i = q;
+        if (step_once(s)) rc = -1;
+    }
+    s->kev = NULL;
+    double te = 0, tu = 0;
+    if (!rc && hipEventSynchronize(ev[3 * nsteps - 1]) != hipSuccess) rc = -1;
+    for (int q = 0; q < nsteps && !rc; q++) {
+        float a = 0, b = 0;
+        if (hipEventElapsedTime(&a, ev[3 * q], ev[3 * q + 1]) != hipSuccess ||
+            hipEventElapsedTime(&b, ev[3 * q + 1], ev[3 * q + 2]) != hipSuccess)
+            rc = -1;
+        te += a;
+        tu += b;
+    }
+    for (int q = 0; q < 3 * nsteps; q++)
+        if (ev[q]) (void)hipEventDestroy(ev[q]);
+    free(ev);
+    if (rc) {
+        if (!sw2_last_error()[0]) sw2_set_error("event timing failed");
+        return -1;
+    }
+    ms[0] = te;
+    ms[1] = tu;
     return 0;
 }
 
