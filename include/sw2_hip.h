@@ -213,7 +213,10 @@ int load_binary_sw(int frame, int *nstep, double *timedouble, char *bname, char 
 
 /* Forms of the device step. */
 #define SW2_FORM_FP64  0   /* plain FP64, same results as SW2's CPU path */
-#define SW2_FORM_MIXED 1   /* FP32 anomalies about an FP64 balanced mean */
+#define SW2_FORM_MIXED 1   /* FP32 anomalies about an FP64 mean state: FP32 fluxes,
+                              * limiters and Laplacians; FP64 mean terms, Adams-Bashforth
+                              * accumulation, Coriolis update and reductions.  Cases the
+                              * fused step takes only (sw2_session_fused). */
 /* Flag or'ed into the form: run one kernel per phase of the step even where
  * the fused two-kernel step applies (same results; for cross-checks). */
 #define SW2_PHASED 0x100
@@ -238,7 +241,13 @@ int sw2_session_time(sw2_session *s, int nsteps, double *ms);
  * kernel (HIP events around every launch on the session's stream; nk >= 2).
  * Fails on a session that steps one kernel per phase. */
 int sw2_session_kernel_times(sw2_session *s, int nsteps, double *ms, int nk);
-/* Copy eta, u, v, tracer, perturbs (halos included) back into SW's arrays. */
+/* SW2's energy diagnostics (energy.c:25-55) and the mass sum H*Dxc*Dyc of
+ * the current state, reduced on the device in FP64 (a fixed-order tree, not
+ * SW2's serial order: for the FP64 form the host's calc_energy_balance and
+ * sw2_mass give SW2's exact sums). */
+int sw2_session_energy(sw2_session *s, double *ke, double *ape, double *mass);
+/* Copy eta, u, v, tracer, perturbs (halos included) back into SW's arrays.
+ * A mixed-form session writes its FP64 mean plus FP32 anomaly. */
 int sw2_session_download(sw2_session *s, sw *SW);
 /* First step whose CFL or stats guard failed (0 if none) and its message. */
 int sw2_session_failed_step(sw2_session *s, char *msg, int msglen);
@@ -248,6 +257,8 @@ int sw2_session_last_cfl(sw2_session *s, double *dtcfl);
  * kernel per phase (in-loop storms or vortices, del-6, active tracer, grids
  * narrower than 128 columns, or SW2_PHASED). */
 int sw2_session_fused(sw2_session *s);
+/* SW2_FORM_FP64 or SW2_FORM_MIXED. */
+int sw2_session_form(sw2_session *s);
 /* Current step index and time. */
 int sw2_session_state(sw2_session *s, int *n, double *t);
 void sw2_session_destroy(sw2_session *s);

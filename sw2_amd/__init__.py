@@ -17,6 +17,7 @@ import numpy as np
 from ._ffi import EXPORTS, LIB_PATH, Params, SW, as_array, last_error, lib  # noqa: F401
 
 FIELDS = ("eta", "u", "v", "tracer", "perturbs", "windx", "windy", "hB")
+FORM_FP64, FORM_MIXED, PHASED = 0, 1, 0x100
 
 
 class SW2Error(RuntimeError):
@@ -142,7 +143,8 @@ def load_state(world, eta, u, v, tracer=None, perturbs=None):
 
 
 class Session:
-    """Device copy of a world; steps run on the GPU (sw2_session_*)."""
+    """Device copy of a world; steps run on the GPU (sw2_session_*).
+    form: FORM_FP64 (SW2's results) or FORM_MIXED (FP32 anomalies)."""
 
     def __init__(self, world, form=0):
         self.world = world
@@ -182,6 +184,16 @@ class Session:
         d = c_double()
         _check(lib().sw2_session_last_cfl(self.h, byref(d)), "sw2_session_last_cfl")
         return d.value
+
+    def energy(self):
+        """(KE, APE, mass) of the current state, reduced on the device in FP64."""
+        ke, ape, m = c_double(), c_double(), c_double()
+        _check(lib().sw2_session_energy(self.h, byref(ke), byref(ape), byref(m)), "sw2_session_energy")
+        return ke.value, ape.value, m.value
+
+    @property
+    def form(self):
+        return lib().sw2_session_form(self.h)
 
     @property
     def fused(self):

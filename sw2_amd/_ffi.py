@@ -81,7 +81,7 @@ EXPORTS = [
     "halo_update", "do_channel", "calc_energy_balance", "sw2_mass",
     "save_binary_sw", "load_binary_sw",
     "sw2_session_create", "sw2_session_begin", "sw2_session_step", "sw2_session_time", "sw2_session_kernel_times",
-    "sw2_session_download",
+    "sw2_session_energy", "sw2_session_download", "sw2_session_form",
     "sw2_session_failed_step", "sw2_session_last_cfl", "sw2_session_fused", "sw2_session_state", "sw2_session_destroy",
     "sw2_last_error", "sw2_build_info",
 ]
@@ -117,6 +117,8 @@ def lib():
     L.sw2_session_time.argtypes = [c_void_p, c_int, P]
     L.sw2_session_kernel_times.argtypes = [c_void_p, c_int, P, c_int]
     L.sw2_session_download.argtypes = [c_void_p, POINTER(SW)]
+    L.sw2_session_energy.argtypes = [c_void_p, P, P, P]
+    L.sw2_session_form.argtypes = [c_void_p]
     L.sw2_session_failed_step.argtypes = [c_void_p, ctypes.c_char_p, c_int]
     L.sw2_session_last_cfl.argtypes = [c_void_p, P]
     L.sw2_session_fused.argtypes = [c_void_p]

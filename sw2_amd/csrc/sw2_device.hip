@@ -20,6 +20,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <utility>
+
 #include "sw2_internal.h"
 #include "sw2_xdiv.h"
 
@@ -506,6 +508,54 @@ __global__ void k_zero(int S, double *f) {
 }  // namespace
 
 #include "sw2_fused.inc"
+#include "sw2_mixed.inc"
+
+namespace {
+// device reductions of SW2's energy diagnostics (energy.c:25-55) and of the
+// mass sum H*Dxc*Dyc over the interior, in FP64, for either form: per-block
+// partial sums, then one block adding them in a fixed order
+template <typename T>
+__global__ __launch_bounds__(256) void k_energy(Grid g, const T *eta, const T *u, const T *v, const double *hbar,
+                                                  double depth, double *part) {
+    double ke = 0, ape = 0, mass = 0;
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < g.nx * g.ny; q += gridDim.x * 256) {
+        const int i = 1 + q % g.nx, j = 1 + q / g.nx;
+        const double wx = TB(g, T_WROW, j);
+        const double H = (double)eta[IX(g, i, j)] + (hbar ? hbar[j + 1] : 0.0) + depth;
+        const double uc = 0.5 * (((double)u[IX(g, i, j)] + wx) + ((double)u[IX(g, i - 1, j)] + wx));
+        const double vc = 0.5 * ((double)v[IX(g, i, j)] + (double)v[IX(g, i, j - 1)]);
+        const double surf = TB(g, SW2_T_DXC, j) * TB(g, SW2_T_DYC, j), gc = TB(g, SW2_T_GC, j);
+        ke += 0.5 * H * (uc * uc + vc * vc) * surf - 0.5 * (wx * wx) * surf;
+        ape += 0.5 * gc * (H * H) * surf - 0.5 * gc * (depth * depth) * surf;
+        mass += H * surf;
+    }
+    __shared__ double r[3][256];
+    r[0][threadIdx.x] = ke;
+    r[1][threadIdx.x] = ape;
+    r[2][threadIdx.x] = mass;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 3; k++) r[k][threadIdx.x] += r[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) part[3 * blockIdx.x + k] = r[k][0];
+}
+
+__global__ void k_energy_fin(int nb, const double *part, double *out) {
+    if (threadIdx.x != 0) return;
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int b = 0; b < nb; b++) {
+        s0 += part[3 * b];
+        s1 += part[3 * b + 1];
+        s2 += part[3 * b + 2];
+    }
+    out[0] = s0;
+    out[1] = s1;
+    out[2] = s2;
+}
+}  // namespace
 
 // ====================================================================== session
 struct sw2_session {
@@ -516,6 +566,11 @@ struct sw2_session {
     int hvmode;     // fused path: 0 none, 2 del-2 and del-4 (del-6 runs phased)
     int cfl_ready;  // CFL maxima of the next step already reduced
     double *eta2, *u2, *v2;  // ping-pong partners of eta, u, v on the fused path
+    // mixed form: FP32 anomaly state, its ping-pong partners and histories;
+    // FP64 mean-state tables (hbar, H0, mean pressure gradient)
+    float *m_eta, *m_u, *m_v, *m_eta2, *m_u2, *m_v2, *m_De[3], *m_Du[3], *m_Dv[3];
+    double *d_mt, *h_hbar;
+    double *d_part;          // energy reduction scratch
     hipEvent_t *kev;         // per-kernel timing (sw2_session_kernel_times): 3 events a step
     int kev_i;
     double t, Dt, depth, sigma, tracer_dis, courant, rest_x, rest_y;
@@ -550,12 +605,75 @@ static void rot(double *h[3]) {
     h[0] = m2;
 }
 
+static int falloc(float **p, size_t n) {
+    HIPCHECK(hipMalloc((void **)p, n * sizeof(float)));
+    HIPCHECK(hipMemset(*p, 0, n * sizeof(float)));
+    return 0;
+}
+
+// the mixed form's mean state and FP32 anomaly fields from the world
+static int mixed_setup(sw2_session *s, sw *SW) {
+    const int R = s->g.R, nx = SW->nx;
+    const size_t S = (size_t)s->S;
+    map *M = SW->M;
+    double *mt = (double *)calloc((size_t)MT_N * R, sizeof(double));
+    s->h_hbar = (double *)calloc((size_t)R, sizeof(double));
+    for (int j = -1; j <= SW->ny + 2; j++) { /* zonal mean of every array row */
+        double m = 0;
+        for (int i = 1; i <= nx; i++) m += ac(SW->eta, i, j, M);
+        s->h_hbar[j + 1] = m / nx;
+    }
+    for (int r = 0; r < R; r++) {
+        mt[MT_HBAR * R + r] = s->h_hbar[r];
+        mt[MT_H0 * R + r] = SW->depth + s->h_hbar[r];
+        if (r + 1 < R) { /* -gn*Dt*(hbar(j+1)-hbar(j))/Dyc(j+1), the mean share of Dv_pres (calc.c:538) */
+            int j = r - 1;
+            mt[MT_P0 * R + r] = -ac(SW->gn, 1, j, M) * SW->Dt * (s->h_hbar[r + 1] - s->h_hbar[r]) / ac(SW->Dyc, 1, j + 1, M);
+        }
+    }
+    int rc = dalloc(&s->d_mt, (size_t)MT_N * R);
+    if (!rc && hipMemcpy(s->d_mt, mt, sizeof(double) * MT_N * R, hipMemcpyHostToDevice) != hipSuccess) rc = -1;
+    free(mt);
+    float **fs[] = {&s->m_eta, &s->m_u, &s->m_v, &s->m_eta2, &s->m_u2, &s->m_v2};
+    for (auto f : fs) rc |= falloc(f, S);
+    for (int k = 0; k < 3; k++) {
+        rc |= falloc(&s->m_De[k], S);
+        rc |= falloc(&s->m_Du[k], S);
+        rc |= falloc(&s->m_Dv[k], S);
+    }
+    if (rc) return -1;
+    float *he = (float *)malloc(sizeof(float) * S), *hu = (float *)malloc(sizeof(float) * S), *hv = (float *)malloc(sizeof(float) * S);
+    for (int j = -1; j <= SW->ny + 2; j++)
+        for (int i = -1; i <= nx + 2; i++) {
+            int q = LP(i, j, M);
+            he[q] = (float)(SW->eta[q] - s->h_hbar[j + 1]);
+            hu[q] = (float)SW->u[q];
+            hv[q] = (float)SW->v[q];
+        }
+    if (hipMemcpy(s->m_eta, he, sizeof(float) * S, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->m_u, hu, sizeof(float) * S, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->m_v, hv, sizeof(float) * S, hipMemcpyHostToDevice) != hipSuccess)
+        rc = -1;
+    free(he);
+    free(hu);
+    free(hv);
+    if (rc) sw2_set_error("upload failed");
+    return rc;
+}
+
+static void frot(float *h[3]) {
+    float *m2 = h[2];
+    h[2] = h[1];
+    h[1] = h[0];
+    h[0] = m2;
+}
+
 extern "C" int sw2_session_create(sw *SW, int form, sw2_session **out) {
     *out = NULL;
     const int force_phased = (form & SW2_PHASED) != 0;
     form &= ~SW2_PHASED;
-    if (form != SW2_FORM_FP64) {
-        sw2_set_error("form %d is not available in this session type", form);
+    if (form != SW2_FORM_FP64 && form != SW2_FORM_MIXED) {
+        sw2_set_error("form %d is not available", form);
         return -1;
     }
     if (SW->polar != 0) {
@@ -621,15 +739,37 @@ extern "C" int sw2_session_create(sw *SW, int form, sw2_session **out) {
                SW->num_vortices == 0 && SW->nu6 == 0 && SW->nx >= 2 * FTX && SW->ny >= 4 &&
                !(SW->nu2 != 0 && SW->nu4 == 0);
     s->hvmode = s->hv ? 2 : 0;
+    if (form == SW2_FORM_MIXED && !s->fused) {
+        sw2_set_error("the mixed-precision form needs a case the fused step takes: no in-loop storms or vortices, "
+                      "no del-6, a zero tracer, a j-only wind, nx >= %d, ny >= 4", 2 * FTX);
+        free(tab);
+        free(s);
+        return -1;
+    }
     rc |= dalloc(&s->d_tab, (size_t)NTD * R);
     if (!rc && hipMemcpy(s->d_tab, tab, sizeof(double) * NTD * R, hipMemcpyHostToDevice) != hipSuccess) rc = -1;
     free(tab);
+    s->g.T = s->d_tab;
+    rc |= dalloc(&s->d_part, 3 * 1024 + 3);
+    if (form == SW2_FORM_MIXED) {
+        if (!rc) rc = mixed_setup(s, SW);
+        if (!rc && hipMalloc((void **)&s->fl, sizeof(Flags)) != hipSuccess) rc = -1;
+        if (!rc && hipMemset(s->fl, 0, sizeof(Flags)) != hipSuccess) rc = -1;
+        if (!rc && hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) rc = -1;
+        if (rc) {
+            if (!sw2_last_error()[0]) sw2_set_error("device allocation failed");
+            sw2_session_destroy(s);
+            return -1;
+        }
+        s->t = SW->t0;
+        *out = s;
+        return 0;
+    }
     if (s->fused) {
         rc |= dalloc(&s->eta2, S);
         rc |= dalloc(&s->u2, S);
         rc |= dalloc(&s->v2, S);
     }
-    s->g.T = s->d_tab;
     double **fields[] = {&s->eta, &s->u, &s->v, &s->tr, &s->pert, &s->wx, &s->wy, &s->duab, &s->dvab};
     for (auto f : fields) rc |= dalloc(f, S);
     for (int k = 0; k < 3; k++) {
@@ -850,7 +990,50 @@ static int step_fused(sw2_session *s) {
 
 static int step_phased(sw2_session *s);
 
-static int step_once(sw2_session *s) { return s->fused ? step_fused(s) : step_phased(s); }
+template <int SCH, int HV>
+static void launch_mixed(sw2_session *s, int n, int k, int slot) {
+    dim3 G((s->g.nx + FTX - 1) / FTX, (s->g.ny + FTY - 1) / FTY);
+    MEtaArgs ea{s->m_eta, s->m_u, s->m_v, s->m_De[1], s->m_De[2], s->m_eta2, s->m_De[0], s->d_mt, n, k, slot,
+                s->Dt, 2.0 * s->Dt * (s->sigma), s->c2, s->c4, s->courant, s->rest_x, s->rest_y};
+    if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i], s->st);
+    km_eta<SCH, HV><<<G, NTE, 0, s->st>>>(s->g, s->fl, ea);
+    if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 1], s->st);
+    MUVArgs ua{s->m_eta2, s->m_u, s->m_v, s->m_Du[1], s->m_Du[2], s->m_Dv[1], s->m_Dv[2], s->m_u2, s->m_v2, s->m_Du[0],
+               s->m_Dv[0], s->d_mt, n, k, slot ^ 1, s->Dt, s->sigma, s->c2, s->c4};
+    km_uv<SCH, HV><<<G, NTU, 0, s->st>>>(s->g, s->fl, ua);
+    if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 2], s->st);
+}
+
+template <int SCH>
+static void launch_mixed_hv(sw2_session *s, int n, int k, int slot) {
+    if (s->hvmode) launch_mixed<SCH, 2>(s, n, k, slot);
+    else launch_mixed<SCH, 0>(s, n, k, slot);
+}
+
+static int step_mixed(sw2_session *s) {
+    const int n = s->n + 1;
+    const int k = n - s->nstart - 1;
+    const int slot = n & 1;
+    if (!s->cfl_ready) {
+        km_cfl<<<1024, 256, 0, s->st>>>(s->g, s->fl, s->m_u, s->m_v, slot);
+        s->cfl_ready = 1;
+    }
+    SCH_SWITCH(launch_mixed_hv, s, n, k, slot);
+    std::swap(s->m_eta, s->m_eta2);
+    std::swap(s->m_u, s->m_u2);
+    std::swap(s->m_v, s->m_v2);
+    frot(s->m_De);
+    frot(s->m_Du);
+    frot(s->m_Dv);
+    s->n = n;
+    s->t = s->t + s->Dt;
+    return 0;
+}
+
+static int step_once(sw2_session *s) {
+    if (s->form == SW2_FORM_MIXED) return step_mixed(s);
+    return s->fused ? step_fused(s) : step_phased(s);
+}
 
 static int step_phased(sw2_session *s) {
     const int n = s->n + 1;
@@ -981,9 +1164,47 @@ extern "C" int sw2_session_kernel_times(sw2_session *s, int nsteps, double *ms, 
     return 0;
 }
 
+extern "C" int sw2_session_energy(sw2_session *s, double *ke, double *ape, double *mass) {
+    const int nb = 1024;
+    double *out = s->d_part + 3 * nb;
+    if (s->form == SW2_FORM_MIXED)
+        k_energy<float><<<nb, 256, 0, s->st>>>(s->g, s->m_eta, s->m_u, s->m_v, s->d_mt + MT_HBAR * s->g.R, s->depth, s->d_part);
+    else
+        k_energy<double><<<nb, 256, 0, s->st>>>(s->g, s->eta, s->u, s->v, nullptr, s->depth, s->d_part);
+    k_energy_fin<<<1, 64, 0, s->st>>>(nb, s->d_part, out);
+    double h[3];
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(h, out, sizeof h, hipMemcpyDeviceToHost, s->st));
+    HIPCHECK(hipStreamSynchronize(s->st));
+    if (ke) *ke = h[0];
+    if (ape) *ape = h[1];
+    if (mass) *mass = h[2];
+    return 0;
+}
+
 extern "C" int sw2_session_download(sw2_session *s, sw *SW) {
     size_t S = (size_t)s->S;
     HIPCHECK(hipStreamSynchronize(s->st));
+    if (s->form == SW2_FORM_MIXED) {
+        float *h = (float *)malloc(sizeof(float) * S);
+        float *src[3] = {s->m_eta, s->m_u, s->m_v};
+        double *dst[3] = {SW->eta, SW->u, SW->v};
+        int rc = 0;
+        for (int f = 0; f < 3 && !rc; f++) {
+            if (hipMemcpy(h, src[f], sizeof(float) * S, hipMemcpyDeviceToHost) != hipSuccess) {
+                rc = -1;
+                break;
+            }
+            for (int j = -1; j <= SW->ny + 2; j++)
+                for (int i = -1; i <= SW->nx + 2; i++) {
+                    int q = LP(i, j, SW->M);
+                    dst[f][q] = (double)h[q] + (f == 0 ? s->h_hbar[j + 1] : 0.0);
+                }
+        }
+        free(h);
+        if (rc) sw2_set_error("download failed");
+        return rc;
+    }
     struct {
         double *d;
         double *h;
@@ -1038,6 +1259,17 @@ extern "C" void sw2_session_destroy(sw2_session *s) {
         if (s->Dtr[k]) (void)hipFree(s->Dtr[k]);
     }
     if (s->pmask) (void)hipFree(s->pmask);
+    float *fall[] = {s->m_eta, s->m_u, s->m_v, s->m_eta2, s->m_u2, s->m_v2};
+    for (auto p : fall)
+        if (p) (void)hipFree(p);
+    for (int k = 0; k < 3; k++) {
+        if (s->m_De[k]) (void)hipFree(s->m_De[k]);
+        if (s->m_Du[k]) (void)hipFree(s->m_Du[k]);
+        if (s->m_Dv[k]) (void)hipFree(s->m_Dv[k]);
+    }
+    if (s->d_mt) (void)hipFree(s->d_mt);
+    if (s->d_part) (void)hipFree(s->d_part);
+    free(s->h_hbar);
     if (s->fl) (void)hipFree(s->fl);
     if (s->st) (void)hipStreamDestroy(s->st);
     for (int k = 0; k < 8; k++) {
@@ -1054,3 +1286,5 @@ extern "C" const char *sw2_build_info(void) {
 }
 
 extern "C" int sw2_session_fused(sw2_session *s) { return s->fused; }
+
+extern "C" int sw2_session_form(sw2_session *s) { return s->form; }
