@@ -20,7 +20,19 @@ Prints one JSON line:
                 measured HBM bytes per launch from the profiler summary in
                 profiles/ (byte counters scaled by a device copy), if present;
   cpu_baseline  the oracle (the CPU restatement of SW2's path) timed on a
-                bounded sample of the same workload on this host.
+                bounded sample of the same workload on this host;
+  mixed         the same grid in the mixed-precision form (FP32 anomalies
+                about an FP64 mean, 60 algorithmic bytes per cell-update) for
+                5000 steps, its speed-up over the FP64 form and its relative
+                L2 / max errors against the FP64 run of the same steps;
+  ensemble      64 members of uneven sizes (256x128 .. 2048x1024, 1-5 seeded
+                storms each) stepped 3000 steps together in the batched mixed
+                form (persistent launches, graph-replayed step, autotuned
+                launch shape): runs/hour, against the FP64 form's runs/hour
+                from its measured per-size step times;
+  served        Poisson arrivals at 4 requests/s of storm runs (512x256 ..
+                4096x2048, 500..20000 steps) through the continuous-batching
+                loop: p50 / p99 time-to-completion and runs/hour.
 """
 import argparse
 import json
@@ -47,6 +59,10 @@ def parse():
     ap.add_argument("--kernel-steps", type=int, default=100, help="steps timed per kernel after the timed region")
     ap.add_argument("--workload", default="c1", choices=["c0", "c1"])
     ap.add_argument("--copy-calib", action="store_true", help="only copy 3 x 512 MiB device to device and exit")
+    ap.add_argument("--mixed-steps", type=int, default=5000, help="steps of the mixed-form run (0: skip)")
+    ap.add_argument("--ensemble-members", type=int, default=64, help="members of the batched ensemble (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=3000)
+    ap.add_argument("--served-requests", type=int, default=40, help="requests of the served stream (0: skip)")
     return ap.parse_args()
 
 
@@ -87,6 +103,115 @@ def cpu_baseline(p_fn, bumps, steps):
     cells = p.nx * p.ny
     return {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
             "sample": f"{p.nx}x{p.ny}, {steps} steps of the same workload, single thread, {dt:.1f} s"}
+
+
+def rel_errors(a, b):
+    import numpy as np
+    d = a - b
+    nb = float(np.linalg.norm(b))
+    mb = float(np.max(np.abs(b)))
+    return {"l2": float(np.linalg.norm(d)) / nb if nb else float(np.linalg.norm(d)),
+            "max": float(np.max(np.abs(d))) / mb if mb else float(np.max(np.abs(d)))}
+
+
+def bench_mixed(steps):
+    """The 4096x2048 workload in both forms for `steps` steps."""
+    from sw2_amd import FORM_FP64, FORM_MIXED, Session, cases
+    p = cases.config_params("c2", steps=steps)
+    bumps = cases.single_bump(2024)
+    res, worlds = {}, {}
+    for form in (FORM_FP64, FORM_MIXED):
+        w, _ = cases.make_world(p, bumps)
+        s = Session(w, form)
+        s.begin(0, p.t0)
+        s.step(2)
+        s.timed_steps(0)
+        t0 = time.perf_counter()
+        dev_ms = s.timed_steps(steps - 2)
+        wall = time.perf_counter() - t0
+        res[form] = {"wall": wall, "dev_ms": dev_ms, "failed": s.failed_step()[0], "energy": s.energy()}
+        s.download()
+        s.close()
+        worlds[form] = w
+    cells = p.nx * p.ny
+    r64, r32 = res[FORM_FP64], res[FORM_MIXED]
+    gbs = 60 * cells / (r32["dev_ms"] / (steps - 2) / 1e3) / 1e9
+    out = {"workload": f"{p.nx}x{p.ny} jet + storm (the FP64 workload's case), {steps} steps",
+           "value": cells * (steps - 2) / r32["wall"], "unit": "cell-updates/s", "dtype": "f32 anomalies / f64 mean",
+           "fp64_value": cells * (steps - 2) / r64["wall"], "speedup_vs_fp64": r64["wall"] / r32["wall"],
+           "bytes_per_cell": 60, "achieved_gbs": gbs, "roofline_frac": gbs / HBM_PEAK_GBS,
+           "failed_step": r32["failed"], "fp64_failed_step": r64["failed"],
+           "errors_vs_fp64": {f: rel_errors(worlds[FORM_MIXED].interior(f), worlds[FORM_FP64].interior(f)) for f in ("eta", "u", "v")},
+           "mass_rel_diff": abs(r32["energy"][2] - r64["energy"][2]) / abs(r64["energy"][2]),
+           "energy_rel_diff": abs((r32["energy"][0] + r32["energy"][1]) - (r64["energy"][0] + r64["energy"][1])) /
+                              abs(r64["energy"][0] + r64["energy"][1])}
+    for w in worlds.values():
+        w.close()
+    return out
+
+
+def fp64_step_ms(nx, ny, steps=40):
+    """Device ms per step of the FP64 form on an nx x ny member case."""
+    import numpy as np
+    from sw2_amd import FORM_FP64, Session, cases
+    p = cases.member_params(nx, ny, steps + 10)
+    w, _ = cases.make_world(p, cases.storm_bumps(np.random.default_rng(nx)))
+    s = Session(w, FORM_FP64)
+    s.begin(0, p.t0)
+    s.step(10)
+    ms = s.timed_steps(steps) / steps
+    s.close()
+    w.close()
+    return ms
+
+
+def bench_ensemble(n, steps):
+    from sw2_amd import Batch, cases, tune
+    specs = cases.ensemble(n, steps)
+    sizes = [(p.nx, p.ny) for p, _ in specs]
+    shape, rec = tune.tune(sizes)
+    b = Batch(capacity=n)
+    b.set_launch(*shape)
+    worlds = []
+    t0 = time.perf_counter()
+    for p, bumps in specs:
+        w, _ = cases.make_world(p, bumps)
+        worlds.append(w)
+        b.add(w, steps)
+    t_admit = time.perf_counter() - t0
+    b.step(shape[2])  # capture the graph (counted steps)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps - shape[2])
+    b.sync()
+    wall = time.perf_counter() - t0
+    fin = b.finished()
+    failed = sum(1 for m in fin if b.member(m)["failed_step"])
+    b.close()
+    for w in worlds:
+        w.close()
+    cells_steps = sum(float(nx) * ny for nx, ny in sizes) * (steps - shape[2])
+    wall_all = wall * steps / (steps - shape[2])
+    ms64 = {sz: fp64_step_ms(*sz) for sz in sorted(set(sizes))}
+    t64 = sum(ms64[sz] for sz in sizes) * steps / 1e3
+    return {"members": n, "steps": steps, "sizes": {f"{a}x{b_}": sizes.count((a, b_)) for a, b_ in sorted(set(sizes))},
+            "runs_per_hour": n / wall_all * 3600.0, "cell_updates_per_s": cells_steps / wall, "wall_s": wall_all,
+            "completed": len(fin), "failed": failed, "launch": {"blocks_eta": shape[0], "blocks_uv": shape[1],
+                                                                   "steps_per_graph": shape[2], "tuning": rec},
+            "admit_s": t_admit, "fp64_ms_per_step": {f"{a}x{b_}": v for (a, b_), v in ms64.items()},
+            "fp64_runs_per_hour": n / t64 * 3600.0, "vs_fp64_runs_per_hour": (n / wall_all) / (n / t64)}
+
+
+def bench_served(n):
+    from sw2_amd import cases, tune
+    from sw2_amd.serve import Server
+    reqs = cases.served_requests(n)
+    srv = Server(capacity=48, hbm_budget=64e9, tick_steps=40)
+    st = srv.run(reqs)
+    srv.close()
+    st.update({"arrivals_per_s": 4.0, "capacity": 48, "hbm_budget_bytes": 64e9, "tick_steps": 40,
+               "sizes": "512x256/1024x512/2048x1024/4096x2048 at 0.4/0.3/0.2/0.1", "steps": "uniform 500..20000"})
+    return st
 
 
 def measured_traffic():
@@ -153,6 +278,13 @@ def main():
         roof = {"bound": "hbm", "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": step_gbs / HBM_PEAK_GBS, "traffic": None,
                 "kernel": "whole step (one kernel per phase)", "bytes_per_cell": STEP_BYTES, "device_ms": step_ms}
+    extras = {}
+    if a.mixed_steps > 0 and wl == "c1":
+        extras["mixed"] = bench_mixed(a.mixed_steps)
+    if a.ensemble_members > 0:
+        extras["ensemble"] = bench_ensemble(a.ensemble_members, a.ensemble_steps)
+    if a.served_requests > 0:
+        extras["served"] = bench_served(a.served_requests)
     out = {
         "metric": "cell-updates/s", "value": value, "unit": "cell-updates/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall * 1e3 / a.steps,
@@ -162,6 +294,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    out.update(extras)
     print(json.dumps(out), flush=True)
 
 

@@ -263,6 +263,47 @@ int sw2_session_form(sw2_session *s);
 int sw2_session_state(sw2_session *s, int *n, double *t);
 void sw2_session_destroy(sw2_session *s);
 
+/* ---------------------------------------------------------------------------
+ * Batches (no counterpart in SW2): many worlds of any sizes stepped together
+ * in the mixed-precision form.  Every active member's tiles go through one
+ * persistent launch per phase; the step is captured once in a hipGraph and
+ * replayed; members join and leave between steps (continuous batching)
+ * without recapture.  Members must be cases the fused step takes and share
+ * the limiter and hyperviscosity order.
+ * ------------------------------------------------------------------------- */
+typedef struct sw2_batch sw2_batch;
+
+/* HBM bytes a member of this size occupies in the given form. */
+size_t sw2_batch_footprint(const sw *SW, int form);
+/* capacity: member slots; hbm_budget_bytes: admission limit on the sum of
+ * member footprints (<= 0: none). */
+int sw2_batch_create(int form, int capacity, double hbm_budget_bytes, sw2_batch **out);
+/* Admit a world (its current state is the initial state) for nsteps steps.
+ * Returns 0 and the member id, 1 when there is no room (slots or HBM
+ * budget; nothing changed), or a negative error. */
+int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id);
+/* Launch shape: persistent workgroups of the height and momentum kernels and
+ * steps per captured graph (the autotuner's knobs). */
+int sw2_batch_set_launch(sw2_batch *b, int blocks_eta, int blocks_uv, int steps_per_graph);
+int sw2_batch_get_launch(sw2_batch *b, int *blocks_eta, int *blocks_uv, int *steps_per_graph);
+/* Queue nsteps steps of every active member (each stops at its own last
+ * step); returns without waiting. */
+int sw2_batch_step(sw2_batch *b, int nsteps);
+int sw2_batch_sync(sw2_batch *b);
+/* Steps completed, whether the member's run ended, first failed step. */
+int sw2_batch_member(sw2_batch *b, int id, int *steps_done, int *done, int *failed_step);
+/* Ids of members whose runs ended; returns how many (<= maxids). */
+int sw2_batch_finished(sw2_batch *b, int *ids, int maxids);
+/* A member's state into SW's arrays (FP64 mean plus FP32 anomaly). */
+int sw2_batch_download(sw2_batch *b, int id, sw *SW);
+/* A member's energy and mass, reduced on the device in FP64. */
+int sw2_batch_energy(sw2_batch *b, int id, double *ke, double *ape, double *mass);
+/* Retire a member and free its slot and device memory. */
+int sw2_batch_remove(sw2_batch *b, int id);
+/* Members admitted and their HBM bytes. */
+int sw2_batch_count(sw2_batch *b, size_t *used_bytes);
+void sw2_batch_destroy(sw2_batch *b);
+
 /* Last error message of this thread's most recent failing sw2_* call. */
 const char *sw2_last_error(void);
 /* Library build string (architecture, flags). */

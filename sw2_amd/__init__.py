@@ -214,3 +214,87 @@ class Session:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Many worlds stepped together in the mixed-precision form
+    (sw2_batch_*): one persistent launch per phase over all members' tiles,
+    the step captured in a hipGraph, members admitted by HBM footprint and
+    retired between steps."""
+
+    def __init__(self, capacity, hbm_budget=0.0, form=FORM_MIXED):
+        self.h = c_void_p()
+        _check(lib().sw2_batch_create(form, capacity, float(hbm_budget), byref(self.h)), "sw2_batch_create")
+        self.worlds = {}
+
+    @staticmethod
+    def footprint(world, form=FORM_MIXED):
+        return lib().sw2_batch_footprint(byref(world.sw), form)
+
+    def add(self, world, nsteps):
+        """Admit a world for nsteps steps; the member id, or None when there
+        is no room (slots or HBM budget)."""
+        mid = c_int(-1)
+        rc = lib().sw2_batch_add(self.h, byref(world.sw), nsteps, byref(mid))
+        if rc == 1:
+            return None
+        _check(rc, "sw2_batch_add")
+        self.worlds[mid.value] = world
+        return mid.value
+
+    def set_launch(self, blocks_eta, blocks_uv, steps_per_graph):
+        _check(lib().sw2_batch_set_launch(self.h, blocks_eta, blocks_uv, steps_per_graph), "sw2_batch_set_launch")
+
+    def launch(self):
+        a, b, c = c_int(), c_int(), c_int()
+        lib().sw2_batch_get_launch(self.h, byref(a), byref(b), byref(c))
+        return a.value, b.value, c.value
+
+    def step(self, n):
+        """Queue n steps of every active member (returns without waiting)."""
+        _check(lib().sw2_batch_step(self.h, n), "sw2_batch_step")
+
+    def sync(self):
+        _check(lib().sw2_batch_sync(self.h), "sw2_batch_sync")
+
+    def member(self, mid):
+        a, b, c = c_int(), c_int(), c_int()
+        _check(lib().sw2_batch_member(self.h, mid, byref(a), byref(b), byref(c)), "sw2_batch_member")
+        return {"steps": a.value, "done": bool(b.value), "failed_step": c.value}
+
+    def finished(self):
+        ids = (c_int * 4096)()
+        n = lib().sw2_batch_finished(self.h, ids, 4096)
+        if n < 0:
+            raise SW2Error(last_error())
+        return list(ids[:n])
+
+    def download(self, mid):
+        w = self.worlds[mid]
+        _check(lib().sw2_batch_download(self.h, mid, byref(w.sw)), "sw2_batch_download")
+        return w
+
+    def energy(self, mid):
+        ke, ape, m = c_double(), c_double(), c_double()
+        _check(lib().sw2_batch_energy(self.h, mid, byref(ke), byref(ape), byref(m)), "sw2_batch_energy")
+        return ke.value, ape.value, m.value
+
+    def remove(self, mid):
+        _check(lib().sw2_batch_remove(self.h, mid), "sw2_batch_remove")
+        self.worlds.pop(mid, None)
+
+    def count(self):
+        used = ctypes.c_size_t()
+        n = lib().sw2_batch_count(self.h, byref(used))
+        return n, used.value
+
+    def close(self):
+        if self.h:
+            lib().sw2_batch_destroy(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -83,6 +83,9 @@ EXPORTS = [
     "sw2_session_create", "sw2_session_begin", "sw2_session_step", "sw2_session_time", "sw2_session_kernel_times",
     "sw2_session_energy", "sw2_session_download", "sw2_session_form",
     "sw2_session_failed_step", "sw2_session_last_cfl", "sw2_session_fused", "sw2_session_state", "sw2_session_destroy",
+    "sw2_batch_footprint", "sw2_batch_create", "sw2_batch_add", "sw2_batch_set_launch", "sw2_batch_get_launch",
+    "sw2_batch_step", "sw2_batch_sync", "sw2_batch_member", "sw2_batch_finished", "sw2_batch_download",
+    "sw2_batch_energy", "sw2_batch_remove", "sw2_batch_count", "sw2_batch_destroy",
     "sw2_last_error", "sw2_build_info",
 ]
 
@@ -124,6 +127,23 @@ def lib():
     L.sw2_session_fused.argtypes = [c_void_p]
     L.sw2_session_state.argtypes = [c_void_p, POINTER(c_int), P]
     L.sw2_session_destroy.argtypes = [c_void_p]
+    PI = POINTER(c_int)
+    L.sw2_batch_footprint.argtypes = [POINTER(SW), c_int]
+    L.sw2_batch_footprint.restype = ctypes.c_size_t
+    L.sw2_batch_create.argtypes = [c_int, c_int, c_double, POINTER(c_void_p)]
+    L.sw2_batch_add.argtypes = [c_void_p, POINTER(SW), c_int, PI]
+    L.sw2_batch_set_launch.argtypes = [c_void_p, c_int, c_int, c_int]
+    L.sw2_batch_get_launch.argtypes = [c_void_p, PI, PI, PI]
+    L.sw2_batch_step.argtypes = [c_void_p, c_int]
+    L.sw2_batch_sync.argtypes = [c_void_p]
+    L.sw2_batch_member.argtypes = [c_void_p, c_int, PI, PI, PI]
+    L.sw2_batch_finished.argtypes = [c_void_p, PI, c_int]
+    L.sw2_batch_download.argtypes = [c_void_p, c_int, POINTER(SW)]
+    L.sw2_batch_energy.argtypes = [c_void_p, c_int, P, P, P]
+    L.sw2_batch_remove.argtypes = [c_void_p, c_int]
+    L.sw2_batch_count.argtypes = [c_void_p, POINTER(ctypes.c_size_t)]
+    L.sw2_batch_destroy.argtypes = [c_void_p]
+    L.sw2_batch_destroy.restype = None
     L.sw2_last_error.restype = ctypes.c_char_p
     L.sw2_build_info.restype = ctypes.c_char_p
     _lib = L

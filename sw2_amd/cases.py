@@ -101,6 +101,54 @@ def initial_state(world, bumps):
     return eta, z, z.copy()
 
 
+# ------------------------------------------------------------ ensembles
+ENSEMBLE_SIZES = [(256, 128), (512, 256), (1024, 512), (2048, 1024)]
+SERVED_SIZES = [(512, 256), (1024, 512), (2048, 1024), (4096, 2048)]
+SERVED_WEIGHTS = [0.4, 0.3, 0.2, 0.1]
+
+
+def storm_bumps(rng):
+    """Seeded storm injections of the ensemble and served workloads: 1-5
+    bumps, amplitude 1-10% of the mean depth, e-fold radius 1-4 deg, centre
+    latitude uniform in -60..60 deg, longitude uniform."""
+    k = int(rng.integers(1, 6))
+    return [(float(rng.uniform(0.0, 360.0)), float(rng.uniform(-60.0, 60.0)), float(rng.uniform(0.01, 0.10)) * DEPTH,
+             float(rng.uniform(1.0, 4.0))) for _ in range(k)]
+
+
+def member_params(nx, ny, steps):
+    """The 360x180 workload's physics on an nx x ny grid (lat -80..80): Dt
+    and the del-4 coefficient scaled with the grid spacing."""
+    return scaled_params(nx, ny, steps=steps, lat=80.0)
+
+
+def ensemble(n=64, steps=3000, seed=2026):
+    """Members of the batched ensemble: grid sizes uniform over
+    ENSEMBLE_SIZES, seeded storm bumps; a list of (params, bumps)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        nx, ny = ENSEMBLE_SIZES[int(rng.integers(0, len(ENSEMBLE_SIZES)))]
+        out.append((member_params(nx, ny, steps), storm_bumps(rng)))
+    return out
+
+
+def served_requests(n, rate=4.0, seed=2027, min_steps=500, max_steps=20000):
+    """Requests of the served stream: Poisson arrivals at `rate` per second,
+    grid sizes from SERVED_SIZES with SERVED_WEIGHTS, step counts uniform in
+    min_steps..max_steps, seeded storm bumps; a list of (t_arrival, nx, ny,
+    steps, bumps)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for _ in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        nx, ny = SERVED_SIZES[int(rng.choice(len(SERVED_SIZES), p=SERVED_WEIGHTS))]
+        steps = int(rng.integers(min_steps, max_steps + 1))
+        out.append((t, nx, ny, steps, storm_bumps(rng)))
+    return out
+
+
 def make_world(params, bumps):
     """World in the state loop() starts from: initial frame loaded."""
     w = World(params)

@@ -1288,3 +1288,351 @@ extern "C" const char *sw2_build_info(void) {
 extern "C" int sw2_session_fused(sw2_session *s) { return s->fused; }
 
 extern "C" int sw2_session_form(sw2_session *s) { return s->form; }
+
+// ====================================================================== batches
+// A batch owns up to `cap` member slots.  Each member is a mixed-form session
+// (its own FP32 state, tables and guard record); the batch steps all of them
+// together with the persistent kernels of sw2_mixed.inc, from a graph of
+// `gsteps` steps captured once.  Members join and leave between steps: a
+// join writes the member's record and zeroes its step count, a leave empties
+// its slot, and neither changes the launches.
+struct sw2_batch {
+    int form, cap, sch, hvmode, nmem;
+    size_t budget, used;
+    sw2_session **sess;
+    size_t *foot;
+    BMember *h_mem;
+    BHdr h_hdr;
+    BMember *d_mem;
+    BState *d_st;
+    BHdr *d_hdr;
+    hipStream_t st;
+    hipGraph_t graph;
+    hipGraphExec_t gexec;
+    int gsteps, blocks_eta, blocks_uv, table_dirty;
+};
+
+extern "C" size_t sw2_batch_footprint(const sw *SW, int form) {
+    const size_t S = (size_t)(SW->nx + 4) * (SW->ny + 4), R = (size_t)SW->ny + 4;
+    size_t f = (size_t)(SW2_NT + 2) * R * sizeof(double) + 3 * R * sizeof(double) + (3 * 1024 + 3) * sizeof(double) + sizeof(Flags);
+    if (form == SW2_FORM_MIXED) f += 15 * S * sizeof(float);
+    else f += 24 * S * sizeof(double);
+    return f;
+}
+
+extern "C" int sw2_batch_create(int form, int capacity, double hbm_budget_bytes, sw2_batch **out) {
+    *out = NULL;
+    if (form != SW2_FORM_MIXED) {
+        sw2_set_error("batches step the mixed-precision form (SW2_FORM_MIXED)");
+        return -1;
+    }
+    if (capacity < 1 || capacity > 4096) {
+        sw2_set_error("capacity %d out of 1..4096", capacity);
+        return -1;
+    }
+    sw2_batch *b = (sw2_batch *)calloc(1, sizeof(sw2_batch));
+    b->form = form;
+    b->cap = capacity;
+    b->sch = -1;
+    b->budget = hbm_budget_bytes > 0 ? (size_t)hbm_budget_bytes : (size_t)-1;
+    b->sess = (sw2_session **)calloc(capacity, sizeof(sw2_session *));
+    b->foot = (size_t *)calloc(capacity, sizeof(size_t));
+    b->h_mem = (BMember *)calloc(capacity, sizeof(BMember));
+    b->gsteps = 8;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || ncu < 1) ncu = 256;
+    b->blocks_eta = 3 * ncu;
+    b->blocks_uv = 4 * ncu;
+    int rc = 0;
+    if (hipMalloc((void **)&b->d_mem, sizeof(BMember) * capacity) != hipSuccess) rc = -1;
+    if (!rc && hipMalloc((void **)&b->d_st, sizeof(BState) * capacity) != hipSuccess) rc = -1;
+    if (!rc && hipMalloc((void **)&b->d_hdr, sizeof(BHdr)) != hipSuccess) rc = -1;
+    if (!rc && hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) rc = -1;
+    if (!rc) {
+        BState *z = (BState *)calloc(capacity, sizeof(BState));
+        for (int m = 0; m < capacity; m++) z[m].done = 1;
+        if (hipMemcpy(b->d_st, z, sizeof(BState) * capacity, hipMemcpyHostToDevice) != hipSuccess) rc = -1;
+        free(z);
+    }
+    b->h_hdr.nslots = capacity;
+    b->h_hdr.total_tiles = 0;
+    b->table_dirty = 1;
+    if (rc) {
+        sw2_set_error("batch allocation failed");
+        sw2_batch_destroy(b);
+        return -1;
+    }
+    *out = b;
+    return 0;
+}
+
+static void batch_retile(sw2_batch *b) {
+    int t = 0;
+    for (int m = 0; m < b->cap; m++) {
+        b->h_mem[m].tile0 = t;
+        t += b->h_mem[m].ntx * b->h_mem[m].nty;
+    }
+    b->h_hdr.total_tiles = t;
+    b->table_dirty = 1;
+}
+
+extern "C" int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id) {
+    if (nsteps < 1) {
+        sw2_set_error("a member needs at least one step");
+        return -1;
+    }
+    int slot = -1;
+    for (int m = 0; m < b->cap && slot < 0; m++)
+        if (!b->sess[m]) slot = m;
+    if (slot < 0) {
+        sw2_set_error("batch full (%d members)", b->cap);
+        return 1;
+    }
+    const size_t foot = sw2_batch_footprint(SW, b->form);
+    if (b->used + foot > b->budget) {
+        sw2_set_error("HBM budget: %zu bytes in use, member needs %zu, budget %zu", b->used, foot, b->budget);
+        return 1;
+    }
+    sw2_session *s = NULL;
+    if (sw2_session_create(SW, b->form, &s)) return -1;
+    if (b->sch < 0) {
+        b->sch = s->sch;
+        b->hvmode = s->hvmode;
+    } else if (s->sch != b->sch || s->hvmode != b->hvmode) {
+        sw2_session_destroy(s);
+        sw2_set_error("members of a batch share the limiter and hyperviscosity order");
+        return -1;
+    }
+    BMember &M = b->h_mem[slot];
+    memset(&M, 0, sizeof M);
+    M.g = s->g;
+    M.fl = s->fl;
+    M.mt = s->d_mt;
+    M.eta[0] = s->m_eta;
+    M.eta[1] = s->m_eta2;
+    M.u[0] = s->m_u;
+    M.u[1] = s->m_u2;
+    M.v[0] = s->m_v;
+    M.v[1] = s->m_v2;
+    for (int k = 0; k < 3; k++) {
+        M.De[k] = s->m_De[k];
+        M.Du[k] = s->m_Du[k];
+        M.Dv[k] = s->m_Dv[k];
+    }
+    M.ntx = (SW->nx + FTX - 1) / FTX;
+    M.nty = (SW->ny + FTY - 1) / FTY;
+    M.nstart = 0;
+    M.nlast = nsteps;
+    M.Dt = s->Dt;
+    M.sig2 = 2.0 * s->Dt * (s->sigma);
+    M.sigma = s->sigma;
+    M.c2 = s->c2;
+    M.c4 = s->c4;
+    M.courant = s->courant;
+    M.rest_x = s->rest_x;
+    M.rest_y = s->rest_y;
+    b->sess[slot] = s;
+    b->foot[slot] = foot;
+    b->used += foot;
+    b->nmem++;
+    batch_retile(b);
+    // a fresh guard record and the CFL maxima of the member's first step
+    BState z{0, 0};
+    HIPCHECK(hipMemsetAsync(s->fl, 0, sizeof(Flags), b->st));
+    km_cfl<<<256, 256, 0, b->st>>>(s->g, s->fl, s->m_u, s->m_v, 1);
+    HIPCHECK(hipMemcpyAsync(b->d_st + slot, &z, sizeof z, hipMemcpyHostToDevice, b->st));
+    HIPCHECK(hipStreamSynchronize(b->st));
+    *id = slot;
+    return 0;
+}
+
+static void batch_launch_step(sw2_batch *b) {
+#define BATCH_LAUNCH(S)                                                                                   \
+    case S:                                                                                               \
+        if (b->hvmode) {                                                                                  \
+            kb_eta<S, 2><<<b->blocks_eta, NTE, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);                \
+            kb_uv<S, 2><<<b->blocks_uv, NTU, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);                  \
+        } else {                                                                                          \
+            kb_eta<S, 0><<<b->blocks_eta, NTE, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);                \
+            kb_uv<S, 0><<<b->blocks_uv, NTU, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);                  \
+        }                                                                                                 \
+        break;
+    switch (b->sch) {
+        BATCH_LAUNCH(0)
+        BATCH_LAUNCH(1)
+        BATCH_LAUNCH(2)
+        BATCH_LAUNCH(3)
+        BATCH_LAUNCH(4)
+        BATCH_LAUNCH(5)
+        BATCH_LAUNCH(6)
+        BATCH_LAUNCH(7)
+        BATCH_LAUNCH(8)
+        default:
+            BATCH_LAUNCH(9)
+    }
+#undef BATCH_LAUNCH
+    kb_adv<<<1, 256, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);
+}
+
+static int batch_upload(sw2_batch *b) {
+    if (!b->table_dirty) return 0;
+    HIPCHECK(hipMemcpyAsync(b->d_mem, b->h_mem, sizeof(BMember) * b->cap, hipMemcpyHostToDevice, b->st));
+    HIPCHECK(hipMemcpyAsync(b->d_hdr, &b->h_hdr, sizeof(BHdr), hipMemcpyHostToDevice, b->st));
+    HIPCHECK(hipStreamSynchronize(b->st));
+    b->table_dirty = 0;
+    return 0;
+}
+
+static int batch_capture(sw2_batch *b) {
+    if (b->gexec) return 0;
+    HIPCHECK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+    for (int q = 0; q < b->gsteps; q++) batch_launch_step(b);
+    HIPCHECK(hipStreamEndCapture(b->st, &b->graph));
+    HIPCHECK(hipGraphInstantiate(&b->gexec, b->graph, NULL, NULL, 0));
+    return 0;
+}
+
+extern "C" int sw2_batch_set_launch(sw2_batch *b, int blocks_eta, int blocks_uv, int steps_per_graph) {
+    if (blocks_eta < 1 || blocks_uv < 1 || steps_per_graph < 1 || steps_per_graph > 256) {
+        sw2_set_error("launch shape out of range");
+        return -1;
+    }
+    HIPCHECK(hipStreamSynchronize(b->st));
+    if (b->gexec) {
+        (void)hipGraphExecDestroy(b->gexec);
+        (void)hipGraphDestroy(b->graph);
+        b->gexec = NULL;
+        b->graph = NULL;
+    }
+    b->blocks_eta = blocks_eta;
+    b->blocks_uv = blocks_uv;
+    b->gsteps = steps_per_graph;
+    return 0;
+}
+
+extern "C" int sw2_batch_get_launch(sw2_batch *b, int *blocks_eta, int *blocks_uv, int *steps_per_graph) {
+    if (blocks_eta) *blocks_eta = b->blocks_eta;
+    if (blocks_uv) *blocks_uv = b->blocks_uv;
+    if (steps_per_graph) *steps_per_graph = b->gsteps;
+    return 0;
+}
+
+// advance every active member by nsteps (members stop at their own last
+// step): whole graphs of gsteps steps, the remainder launched directly
+extern "C" int sw2_batch_step(sw2_batch *b, int nsteps) {
+    if (b->nmem == 0 || nsteps <= 0) return 0;
+    if (batch_upload(b)) return -1;
+    if (batch_capture(b)) return -1;
+    int q = 0;
+    for (; q + b->gsteps <= nsteps; q += b->gsteps) HIPCHECK(hipGraphLaunch(b->gexec, b->st));
+    for (; q < nsteps; q++) batch_launch_step(b);
+    HIPCHECK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int sw2_batch_sync(sw2_batch *b) {
+    HIPCHECK(hipStreamSynchronize(b->st));
+    return 0;
+}
+
+extern "C" int sw2_batch_member(sw2_batch *b, int id, int *steps_done, int *done, int *failed_step) {
+    if (id < 0 || id >= b->cap || !b->sess[id]) {
+        sw2_set_error("no member %d", id);
+        return -1;
+    }
+    BState S;
+    HIPCHECK(hipMemcpyAsync(&S, b->d_st + id, sizeof S, hipMemcpyDeviceToHost, b->st));
+    HIPCHECK(hipStreamSynchronize(b->st));
+    if (steps_done) *steps_done = S.nstep;
+    if (done) *done = S.done;
+    if (failed_step) *failed_step = sw2_session_failed_step(b->sess[id], NULL, 0);
+    return 0;
+}
+
+// ids of the members whose runs ended (last step reached or a guard failed)
+extern "C" int sw2_batch_finished(sw2_batch *b, int *ids, int maxids) {
+    BState *h = (BState *)malloc(sizeof(BState) * b->cap);
+    if (hipMemcpyAsync(h, b->d_st, sizeof(BState) * b->cap, hipMemcpyDeviceToHost, b->st) != hipSuccess ||
+        hipStreamSynchronize(b->st) != hipSuccess) {
+        free(h);
+        sw2_set_error("reading member states failed");
+        return -1;
+    }
+    int n = 0;
+    for (int m = 0; m < b->cap; m++)
+        if (b->sess[m] && h[m].done && n < maxids) ids[n++] = m;
+    free(h);
+    return n;
+}
+
+extern "C" int sw2_batch_download(sw2_batch *b, int id, sw *SW) {
+    int steps = 0;
+    if (sw2_batch_member(b, id, &steps, NULL, NULL)) return -1;
+    sw2_session *s = b->sess[id];
+    const BMember &M = b->h_mem[id];
+    const int p = (steps - M.nstart) & 1;  // buffer holding the state after `steps` steps
+    s->m_eta = M.eta[p];
+    s->m_eta2 = M.eta[p ^ 1];
+    s->m_u = M.u[p];
+    s->m_u2 = M.u[p ^ 1];
+    s->m_v = M.v[p];
+    s->m_v2 = M.v[p ^ 1];
+    s->n = steps;
+    s->t = SW->t0 + steps * s->Dt;
+    return sw2_session_download(s, SW);
+}
+
+extern "C" int sw2_batch_energy(sw2_batch *b, int id, double *ke, double *ape, double *mass) {
+    int steps = 0;
+    if (sw2_batch_member(b, id, &steps, NULL, NULL)) return -1;
+    sw2_session *s = b->sess[id];
+    const BMember &M = b->h_mem[id];
+    const int p = (steps - M.nstart) & 1;
+    s->m_eta = M.eta[p];
+    s->m_u = M.u[p];
+    s->m_v = M.v[p];
+    return sw2_session_energy(s, ke, ape, mass);
+}
+
+extern "C" int sw2_batch_remove(sw2_batch *b, int id) {
+    if (id < 0 || id >= b->cap || !b->sess[id]) {
+        sw2_set_error("no member %d", id);
+        return -1;
+    }
+    HIPCHECK(hipStreamSynchronize(b->st));
+    BState z{0, 1};
+    HIPCHECK(hipMemcpy(b->d_st + id, &z, sizeof z, hipMemcpyHostToDevice));
+    memset(&b->h_mem[id], 0, sizeof(BMember));
+    batch_retile(b);
+    // the member's device arrays are released only after the table that
+    // still names them has been replaced
+    if (batch_upload(b)) return -1;
+    sw2_session_destroy(b->sess[id]);
+    b->sess[id] = NULL;
+    b->used -= b->foot[id];
+    b->foot[id] = 0;
+    b->nmem--;
+    return 0;
+}
+
+extern "C" int sw2_batch_count(sw2_batch *b, size_t *used_bytes) {
+    if (used_bytes) *used_bytes = b->used;
+    return b->nmem;
+}
+
+extern "C" void sw2_batch_destroy(sw2_batch *b) {
+    if (!b) return;
+    if (b->st) (void)hipStreamSynchronize(b->st);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (int m = 0; m < b->cap; m++)
+        if (b->sess && b->sess[m]) sw2_session_destroy(b->sess[m]);
+    if (b->d_mem) (void)hipFree(b->d_mem);
+    if (b->d_st) (void)hipFree(b->d_st);
+    if (b->d_hdr) (void)hipFree(b->d_hdr);
+    if (b->st) (void)hipStreamDestroy(b->st);
+    free(b->sess);
+    free(b->foot);
+    free(b->h_mem);
+    free(b);
+}

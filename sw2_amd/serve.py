@@ -1,0 +1,90 @@
+"""Serving SW2 runs: a continuous-batching request loop over sw2_batch_*.
+
+Requests (arrival time, grid, steps, storm bumps) arrive on their own
+schedule.  The loop admits a waiting request as soon as a member slot and
+the HBM budget allow (sw2_batch_add checks the member's footprint), keeps
+every admitted run's state on the device, advances all of them together in
+ticks of a few steps (one persistent launch per phase for every member,
+replayed from the captured step graph), and retires a run the tick it ends.
+Requests that arrive while others are on the GPU join the running batch at
+the next tick; their work overlaps with the runs already there.
+
+Reported: time-to-completion (arrival to retirement) p50 / p99, runs per
+hour, cell-updates per second and the fraction of time the device had work.
+"""
+import time
+from collections import deque
+
+import numpy as np
+
+from . import Batch, cases
+
+
+class Server:
+    def __init__(self, capacity=64, hbm_budget=0.0, tick_steps=40, launch=None):
+        self.batch = Batch(capacity=capacity, hbm_budget=hbm_budget)
+        if launch:
+            self.batch.set_launch(*launch)
+        self.tick = tick_steps
+
+    def close(self):
+        self.batch.close()
+
+    def run(self, requests, on_result=None, max_wall=None):
+        """Serve `requests` = [(t_arrival_s, nx, ny, steps, bumps), ...] in
+        real time; returns the statistics (and calls on_result(k, batch,
+        member_id) before each run is retired, for callers that keep the
+        results)."""
+        reqs = sorted(enumerate(requests), key=lambda r: r[1][0])
+        waiting = deque()
+        live = {}  # member id -> (request index, arrival, admitted)
+        done = {}
+        cells_steps = 0.0
+        busy = 0.0
+        t0 = time.perf_counter()
+        nxt = 0
+        while nxt < len(reqs) or waiting or live:
+            now = time.perf_counter() - t0
+            if max_wall and now > max_wall:
+                break
+            while nxt < len(reqs) and reqs[nxt][1][0] <= now:
+                waiting.append(reqs[nxt])
+                nxt += 1
+            while waiting:
+                k, (ta, nx, ny, steps, bumps) = waiting[0]
+                w, _ = cases.make_world(cases.member_params(nx, ny, steps), bumps)
+                mid = self.batch.add(w, steps)
+                if mid is None:
+                    w.close()
+                    break
+                waiting.popleft()
+                live[mid] = (k, ta, time.perf_counter() - t0)
+                cells_steps += float(nx) * ny * steps
+            if not live:
+                if nxt < len(reqs):
+                    time.sleep(max(0.0, min(0.05, reqs[nxt][1][0] - (time.perf_counter() - t0))))
+                continue
+            tb = time.perf_counter()
+            self.batch.step(self.tick)
+            fin = self.batch.finished()  # waits for the tick
+            busy += time.perf_counter() - tb
+            tnow = time.perf_counter() - t0
+            for mid in fin:
+                k, ta, tadm = live.pop(mid)
+                st = self.batch.member(mid)
+                if on_result:
+                    on_result(k, self.batch, mid)
+                done[k] = {"arrival": ta, "admitted": tadm, "completed": tnow, "steps": st["steps"],
+                           "failed_step": st["failed_step"]}
+                self.batch.remove(mid)
+        wall = time.perf_counter() - t0
+        lat = np.array([d["completed"] - d["arrival"] for d in done.values()]) if done else np.array([np.nan])
+        wait = np.array([d["admitted"] - d["arrival"] for d in done.values()]) if done else np.array([np.nan])
+        return {
+            "requests": len(requests), "completed": len(done),
+            "failed": sum(1 for d in done.values() if d["failed_step"]),
+            "wall_s": wall, "runs_per_hour": len(done) / wall * 3600.0,
+            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "queue_wait_p50_s": float(np.percentile(wait, 50)),
+            "cell_updates_per_s": cells_steps / wall, "device_busy_frac": busy / wall,
+        }

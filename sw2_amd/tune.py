@@ -1,0 +1,97 @@
+"""Autotuner of the batched step's launch shape.
+
+For a mix of member grid sizes it builds a scratch batch of the same
+composition (members at rest: the mixed-form kernels take the same path
+whatever the data), times whole steps for candidate launch shapes and keeps
+the fastest.  The knobs are the persistent workgroups of the height kernel
+and of the momentum kernel (multiples of the device's compute-unit count)
+and the steps per captured graph; the search is coordinate descent from the
+occupancy-derived default.  Results are cached per device and mix in a JSON
+file (default: tuned_launch.json beside this module, or $SW2_TUNE_CACHE).
+"""
+import json
+import os
+import time
+from collections import Counter
+
+from . import Batch, cases
+
+CACHE = os.environ.get("SW2_TUNE_CACHE", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_launch.json"))
+
+
+def device_cus():
+    """Compute units of the device (a new batch's default height-kernel grid
+    is three workgroups per compute unit)."""
+    b = Batch(capacity=1)
+    n = b.launch()[0] // 3
+    b.close()
+    return n
+
+
+def mix_key(sizes, ncu):
+    c = Counter(sizes)
+    return f"cu{ncu}|mixed|" + ",".join(f"{nx}x{ny}:{k}" for (nx, ny), k in sorted(c.items()))
+
+
+def _time_steps(b, shape, steps):
+    b.set_launch(*shape)
+    b.step(shape[2])  # capture and warm up
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def tune(sizes, steps=16, cache=CACHE, verbose=False):
+    """Best (blocks_eta, blocks_uv, steps_per_graph) for members of `sizes`
+    ([(nx, ny), ...]); returns (shape, record)."""
+    ncu = device_cus()
+    key = mix_key(sizes, ncu)
+    db = {}
+    if cache and os.path.exists(cache):
+        try:
+            with open(cache) as f:
+                db = json.load(f)
+        except (OSError, ValueError):
+            db = {}
+    if key in db:
+        r = db[key]
+        return tuple(r["shape"]), r
+    b = Batch(capacity=len(sizes))
+    worlds = []
+    for nx, ny in sizes:
+        w, _ = cases.make_world(cases.member_params(nx, ny, 1 << 30), [])
+        worlds.append(w)
+        b.add(w, 1 << 30)
+    t_start = time.perf_counter()
+    best = (3 * ncu, 4 * ncu, 8)
+    best_t = _time_steps(b, best, steps)
+    tried = {best: best_t}
+    for axis, options in ((0, [1, 2, 3, 4, 6]), (1, [1, 2, 3, 4, 6]), (2, [4, 8, 16])):
+        for o in options:
+            shape = list(best)
+            shape[axis] = o * ncu if axis < 2 else o
+            shape = tuple(shape)
+            if shape in tried:
+                continue
+            t = _time_steps(b, shape, steps)
+            tried[shape] = t
+            if verbose:
+                print("tune", shape, f"{t * 1e3:.3f} ms/step", flush=True)
+            if t < best_t:
+                best, best_t = shape, t
+    b.close()
+    for w in worlds:
+        w.close()
+    rec = {"shape": list(best), "ms_per_step": best_t * 1e3, "candidates": len(tried),
+           "tuning_s": time.perf_counter() - t_start,
+           "default_ms_per_step": tried[(3 * ncu, 4 * ncu, 8)] * 1e3}
+    db[key] = rec
+    if cache:
+        try:
+            with open(cache, "w") as f:
+                json.dump(db, f, indent=1)
+        except OSError:
+            pass
+    return best, rec

@@ -1,0 +1,117 @@
+"""Batches: worlds of uneven sizes stepped together (one persistent launch
+per phase, the step replayed from a captured hipGraph, members joining and
+leaving between steps).  Every member's result must be the one the same
+world gives alone in a mixed-precision session -- bit for bit, since the
+member runs the same arithmetic -- and, through that, within the mixed
+form's tolerance of the FP64 oracle (tests/test_gpu_mixed.py)."""
+import numpy as np
+import pytest
+
+from sw2_amd import Batch, FORM_MIXED, Session, cases
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("eta", "u", "v")
+
+
+def solo(p, bumps, n):
+    w, _ = cases.make_world(p, bumps)
+    s = Session(w, FORM_MIXED)
+    s.begin(0, p.t0)
+    s.step(n)
+    assert s.failed_step()[0] == 0
+    s.download()
+    return w
+
+
+def same(a, b, label):
+    for f in FIELDS:
+        x, y = a.field(f), b.field(f)
+        assert np.array_equal(x, y), (label, f, int(np.sum(x != y)))
+
+
+def small_members(seed, n):
+    rng = np.random.default_rng(seed)
+    sizes = [(128, 64), (256, 128), (192, 40), (512, 256)]
+    out = []
+    for k in range(n):
+        nx, ny = sizes[k % len(sizes)]
+        steps = int(rng.integers(20, 60))
+        out.append((cases.member_params(nx, ny, steps), cases.storm_bumps(rng), steps))
+    return out
+
+
+def test_batch_members_equal_solo_runs():
+    members = small_members(1, 6)
+    b = Batch(capacity=8)
+    ids = []
+    for p, bumps, steps in members:
+        w, _ = cases.make_world(p, bumps)
+        ids.append(b.add(w, steps))
+    print("launch", b.launch(), flush=True)
+    b.step(max(m[2] for m in members) + 5)  # members stop at their own last step
+    b.sync()
+    for mid, (p, bumps, steps) in zip(ids, members):
+        st = b.member(mid)
+        assert st["done"] and st["steps"] == steps and st["failed_step"] == 0, st
+        same(b.download(mid), solo(p, bumps, steps), f"member {mid}")
+    assert sorted(b.finished()) == sorted(ids)
+    b.close()
+
+
+def test_graph_and_direct_launches_agree():
+    p = cases.member_params(256, 128, 40)
+    bumps = cases.storm_bumps(np.random.default_rng(3))
+    ref = solo(p, bumps, 37)
+    b = Batch(capacity=2)
+    b.set_launch(97, 131, 5)  # odd persistent grids; 37 = 7 graphs of 5 + 2 direct steps
+    w, _ = cases.make_world(p, bumps)
+    mid = b.add(w, 37)
+    b.step(37)
+    b.sync()
+    assert b.member(mid)["steps"] == 37
+    same(b.download(mid), ref, "graph+direct")
+    b.close()
+
+
+def test_continuous_batching_join_and_leave():
+    """Members join while others run and leave when done; each result is
+    the solo result."""
+    members = small_members(7, 7)
+    b = Batch(capacity=4)
+    pending = list(enumerate(members))
+    live = {}
+    results = {}
+    ticks = 0
+    while pending or live:
+        while pending:
+            k, (p, bumps, steps) = pending[0]
+            w, _ = cases.make_world(p, bumps)
+            mid = b.add(w, steps)
+            if mid is None:
+                break
+            live[mid] = k
+            pending.pop(0)
+        b.step(7)
+        ticks += 1
+        for mid in b.finished():
+            k = live.pop(mid)
+            results[k] = b.download(mid)
+            b.remove(mid)
+        assert ticks < 200
+    for k, (p, bumps, steps) in enumerate(members):
+        same(results[k], solo(p, bumps, steps), f"request {k}")
+    b.close()
+
+
+def test_admission_by_hbm_footprint():
+    p = cases.member_params(512, 256, 10)
+    w, _ = cases.make_world(p, [])
+    foot = Batch.footprint(w)
+    b = Batch(capacity=8, hbm_budget=2.5 * foot)
+    ids = [b.add(cases.make_world(p, [])[0], 10) for _ in range(3)]
+    assert ids[0] is not None and ids[1] is not None and ids[2] is None
+    assert b.count() == (2, 2 * foot)
+    b.remove(ids[0])
+    assert b.add(cases.make_world(p, [])[0], 10) is not None
+    b.close()
