@@ -105,6 +105,11 @@ def cpu_baseline(p_fn, bumps, steps):
             "sample": f"{p.nx}x{p.ny}, {steps} steps of the same workload, single thread, {dt:.1f} s"}
 
 
+def progress(msg):
+    """A progress line on stderr (stdout carries only the JSON result)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def rel_errors(a, b):
     import numpy as np
     d = a - b
@@ -121,6 +126,7 @@ def bench_mixed(steps):
     bumps = cases.single_bump(2024)
     res, worlds = {}, {}
     for form in (FORM_FP64, FORM_MIXED):
+        progress(f"mixed-form comparison: form {form}, {steps} steps")
         w, _ = cases.make_world(p, bumps)
         s = Session(w, form)
         s.begin(0, p.t0)
@@ -169,7 +175,9 @@ def bench_ensemble(n, steps):
     from sw2_amd import Batch, cases, tune
     specs = cases.ensemble(n, steps)
     sizes = [(p.nx, p.ny) for p, _ in specs]
+    progress(f"ensemble: tuning the launch shape for {n} members")
     shape, rec = tune.tune(sizes)
+    progress(f"ensemble: launch {shape}; admitting {n} members")
     b = Batch(capacity=n)
     b.set_launch(*shape)
     worlds = []
@@ -179,6 +187,7 @@ def bench_ensemble(n, steps):
         worlds.append(w)
         b.add(w, steps)
     t_admit = time.perf_counter() - t0
+    progress(f"ensemble: {steps} steps")
     b.step(shape[2])  # capture the graph (counted steps)
     b.sync()
     t0 = time.perf_counter()
@@ -192,6 +201,7 @@ def bench_ensemble(n, steps):
         w.close()
     cells_steps = sum(float(nx) * ny for nx, ny in sizes) * (steps - shape[2])
     wall_all = wall * steps / (steps - shape[2])
+    progress("ensemble: FP64 step times per member size")
     ms64 = {sz: fp64_step_ms(*sz) for sz in sorted(set(sizes))}
     t64 = sum(ms64[sz] for sz in sizes) * steps / 1e3
     return {"members": n, "steps": steps, "sizes": {f"{a}x{b_}": sizes.count((a, b_)) for a, b_ in sorted(set(sizes))},
@@ -206,8 +216,9 @@ def bench_served(n):
     from sw2_amd import cases, tune
     from sw2_amd.serve import Server
     reqs = cases.served_requests(n)
+    progress(f"served stream: {n} requests")
     srv = Server(capacity=48, hbm_budget=64e9, tick_steps=40)
-    st = srv.run(reqs)
+    st = srv.run(reqs, log=progress)
     srv.close()
     st.update({"arrivals_per_s": 4.0, "capacity": 48, "hbm_budget_bytes": 64e9, "tick_steps": 40,
                "sizes": "512x256/1024x512/2048x1024/4096x2048 at 0.4/0.3/0.2/0.1", "steps": "uniform 500..20000"})
@@ -232,6 +243,7 @@ def main():
         return
     from sw2_amd import Session, cases
 
+    progress("FP64 workload")
     wl = a.workload
     total = a.warmup + a.steps + a.kernel_steps
     p_fn = lambda n: cases.config_params(wl, steps=n)
@@ -258,6 +270,8 @@ def main():
     fused = s.fused
     s.close()
     w.close()
+    if a.cpu_steps > 0:
+        progress(f"CPU baseline: {a.cpu_steps} oracle steps")
     cpu = cpu_baseline(p_fn, bumps, a.cpu_steps) if a.cpu_steps > 0 else None
     names = {"c1": "4096x2048 FP64 jet + storm, lat -85..85, Dt 10 s, del-4 5.97e10, minmod",
              "c0": "360x180 FP64 jet + storm, lat -80..80, Dt 60 s, del-4 1e15, minmod"}

@@ -30,7 +30,7 @@ class Server:
     def close(self):
         self.batch.close()
 
-    def run(self, requests, on_result=None, max_wall=None):
+    def run(self, requests, on_result=None, max_wall=None, log=None):
         """Serve `requests` = [(t_arrival_s, nx, ny, steps, bumps), ...] in
         real time; returns the statistics (and calls on_result(k, batch,
         member_id) before each run is retired, for callers that keep the
@@ -69,6 +69,9 @@ class Server:
             fin = self.batch.finished()  # waits for the tick
             busy += time.perf_counter() - tb
             tnow = time.perf_counter() - t0
+            if log and fin:
+                log(f"t={tnow:.1f}s: {len(fin)} done, {len(done) + len(fin)}/{len(requests)} completed, "
+                    f"{len(live) - len(fin)} running, {len(waiting)} waiting")
             for mid in fin:
                 k, ta, tadm = live.pop(mid)
                 st = self.batch.member(mid)

@@ -63,24 +63,16 @@ def test_mixed_all_limiters(sch):
     p = cases.scaled_params(128, 24, steps=40, lat=70.0, sch=sch)
     n = 40
     w, o, s, fs, fo = run_pair(p, cases.single_bump(sch), n)
-    if fo:
-        # SW2 itself goes wrong here: van Leer's (r+|r|)/(1+r) is 0/0 at
-        # r = -1 (calc.c:57); the NaN spreads until the checks see it (they
-        # look for NaN only at cell (1,1), sppde_core.c:346-395).  The mixed
-        # form takes the limiter's value 0 there; compare up to the last
-        # step SW2's state is still free of NaN.
-        assert sch == 3 and fs == 0, (sch, fo, fs)
-        w0, (eta, u, v) = cases.make_world(p, cases.single_bump(sch))
-        o0 = Oracle(p)
-        o0.load(eta, u, v)
-        n = 0
-        while n < fo:
-            o0.step(1)
-            if any(np.isnan(o0.interior(f)).any() for f in ("eta", "u", "v")):
-                break
-            n += 1
-        assert n >= 1, "SW2 has NaN after the first step"
-        w, o, s, fs, fo = run_pair(p, cases.single_bump(sch), n)
+    if sch == 3:
+        # Van Leer's (r+|r|)/(1+r) is 0/0 at r = -1 (calc.c:57): on this jet
+        # case SW2 turns NaN in the first step and its checks stop it at
+        # step 3 (the FP64 parity tests reproduce that NaN bit for bit).  The
+        # mixed form takes the limiter's value 0 at r = -1 and runs on: there
+        # is no SW2 result to compare it with.
+        assert fo > 0 and fs == 0, (fo, fs)
+        for f in ("eta", "u", "v"):
+            assert np.isfinite(w.interior(f)).all()
+        return
     assert fs == fo == 0
     e = report(f"mixed sch={sch} 128x24 {n} steps", w, o.interior)
     for f, (l2, mx) in e.items():
