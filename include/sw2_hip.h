@@ -273,6 +273,8 @@ void sw2_session_destroy(sw2_session *s);
  * ------------------------------------------------------------------------- */
 typedef struct sw2_batch sw2_batch;
 
+/* Compute units of the current device (the unit of the launch shapes). */
+int sw2_device_cus(void);
 /* HBM bytes a member of this size occupies in the given form. */
 size_t sw2_batch_footprint(const sw *SW, int form);
 /* capacity: member slots; hbm_budget_bytes: admission limit on the sum of
@@ -283,7 +285,9 @@ int sw2_batch_create(int form, int capacity, double hbm_budget_bytes, sw2_batch 
  * budget; nothing changed), or a negative error. */
 int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id);
 /* Launch shape: persistent workgroups of the height and momentum kernels and
- * steps per captured graph (the autotuner's knobs). */
+ * steps per captured graph (the autotuner's knobs).  A new batch starts
+ * with the workgroups that stay resident at once on the device and 8 steps
+ * per graph. */
 int sw2_batch_set_launch(sw2_batch *b, int blocks_eta, int blocks_uv, int steps_per_graph);
 int sw2_batch_get_launch(sw2_batch *b, int *blocks_eta, int *blocks_uv, int *steps_per_graph);
 /* Queue nsteps steps of every active member (each stops at its own last

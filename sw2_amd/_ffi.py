@@ -83,7 +83,7 @@ EXPORTS = [
     "sw2_session_create", "sw2_session_begin", "sw2_session_step", "sw2_session_time", "sw2_session_kernel_times",
     "sw2_session_energy", "sw2_session_download", "sw2_session_form",
     "sw2_session_failed_step", "sw2_session_last_cfl", "sw2_session_fused", "sw2_session_state", "sw2_session_destroy",
-    "sw2_batch_footprint", "sw2_batch_create", "sw2_batch_add", "sw2_batch_set_launch", "sw2_batch_get_launch",
+    "sw2_device_cus", "sw2_batch_footprint", "sw2_batch_create", "sw2_batch_add", "sw2_batch_set_launch", "sw2_batch_get_launch",
     "sw2_batch_step", "sw2_batch_sync", "sw2_batch_member", "sw2_batch_finished", "sw2_batch_download",
     "sw2_batch_energy", "sw2_batch_remove", "sw2_batch_count", "sw2_batch_destroy",
     "sw2_last_error", "sw2_build_info",
@@ -128,6 +128,8 @@ def lib():
     L.sw2_session_state.argtypes = [c_void_p, POINTER(c_int), P]
     L.sw2_session_destroy.argtypes = [c_void_p]
     PI = POINTER(c_int)
+    L.sw2_device_cus.argtypes = []
+    L.sw2_device_cus.restype = c_int
     L.sw2_batch_footprint.argtypes = [POINTER(SW), c_int]
     L.sw2_batch_footprint.restype = ctypes.c_size_t
     L.sw2_batch_create.argtypes = [c_int, c_int, c_double, POINTER(c_void_p)]

@@ -1320,6 +1320,12 @@ extern "C" size_t sw2_batch_footprint(const sw *SW, int form) {
     return f;
 }
 
+extern "C" int sw2_device_cus(void) {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || ncu < 1) ncu = 256;
+    return ncu;
+}
+
 extern "C" int sw2_batch_create(int form, int capacity, double hbm_budget_bytes, sw2_batch **out) {
     *out = NULL;
     if (form != SW2_FORM_MIXED) {
@@ -1339,10 +1345,15 @@ extern "C" int sw2_batch_create(int form, int capacity, double hbm_budget_bytes,
     b->foot = (size_t *)calloc(capacity, sizeof(size_t));
     b->h_mem = (BMember *)calloc(capacity, sizeof(BMember));
     b->gsteps = 8;
-    int ncu = 0;
-    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || ncu < 1) ncu = 256;
-    b->blocks_eta = 3 * ncu;
-    b->blocks_uv = 4 * ncu;
+    // default persistent grids: as many workgroups as stay resident at once
+    // (occupancy of the minmod, del-4 instantiations), so that the static
+    // tile stride of the kernels never leaves a second, partial round
+    const int ncu = sw2_device_cus();
+    int occ_e = 0, occ_u = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_e, kb_eta<2, 2>, NTE, 0) != hipSuccess || occ_e < 1) occ_e = 2;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_u, kb_uv<2, 2>, NTU, 0) != hipSuccess || occ_u < 1) occ_u = 3;
+    b->blocks_eta = occ_e * ncu;
+    b->blocks_uv = occ_u * ncu;
     int rc = 0;
     if (hipMalloc((void **)&b->d_mem, sizeof(BMember) * capacity) != hipSuccess) rc = -1;
     if (!rc && hipMalloc((void **)&b->d_st, sizeof(BState) * capacity) != hipSuccess) rc = -1;

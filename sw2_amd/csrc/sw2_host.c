@@ -537,17 +537,17 @@ int sw2_init_params(const sw2_params *P, sw *SW) {
             x[r] = P->wind[2 * r];
             y[r] = P->wind[2 * r + 1];
         }
-        for (int j = -1; j <= SW->ny + 2; j++)
-            for (int i = -1; i <= SW->nx + 2; i++) {
-                double w;
-                if (linterp_host(RAD2DEG(g_latc(SW, j)), x, y, nr, &w)) {
-                    sw2_set_error("Latitude value %e out of bounds of the wind table (%e, %e)", RAD2DEG(g_latc(SW, j)), x[0], x[nr - 1]);
-                    free(x);
-                    free(y);
-                    return -1;
-                }
-                ac(SW->windx, i, j, SW->M) = w;
+        /* the interpolant depends on the row only: one table search per row */
+        for (int j = -1; j <= SW->ny + 2; j++) {
+            double w;
+            if (linterp_host(RAD2DEG(g_latc(SW, j)), x, y, nr, &w)) {
+                sw2_set_error("Latitude value %e out of bounds of the wind table (%e, %e)", RAD2DEG(g_latc(SW, j)), x[0], x[nr - 1]);
+                free(x);
+                free(y);
+                return -1;
             }
+            for (int i = -1; i <= SW->nx + 2; i++) ac(SW->windx, i, j, SW->M) = w;
+        }
         free(x);
         free(y);
     }

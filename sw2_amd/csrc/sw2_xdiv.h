@@ -68,10 +68,11 @@ SW2_HD double sw2_from_bits(uint64_t u) {
 }
 
 /* a, b, q finite with magnitudes in [2^-800, 2^800], q within one ulp of
- * a/b: the nearer of q and its neighbour toward a/b (ties to even). */
+ * a/b: the nearer of q and its neighbour toward a/b (ties to even).  No
+ * branch: when q is exact (r == 0) the neighbour's remainder is nonzero and
+ * q is kept by the comparison. */
 SW2_HD double sw2_nearest(double a, double b, double q) {
     double r = fma(-b, q, a);
-    if (r == 0) return q;
     int up = (r > 0) == (b > 0); /* a/b lies above q */
     uint64_t u = sw2_bits(q);
     uint64_t un = ((q > 0) == up) ? u + 1 : u - 1;

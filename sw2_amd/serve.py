@@ -7,36 +7,50 @@ every admitted run's state on the device, advances all of them together in
 ticks of a few steps (one persistent launch per phase for every member,
 replayed from the captured step graph), and retires a run the tick it ends.
 Requests that arrive while others are on the GPU join the running batch at
-the next tick; their work overlaps with the runs already there.
+the next tick; their work overlaps with the runs already there.  A request's
+world (geometry, wind table, storm bumps) is built on the host by a small
+pool of builder threads as soon as it arrives, while the device runs the
+current tick, so admission only uploads it.
 
 Reported: time-to-completion (arrival to retirement) p50 / p99, runs per
 hour, cell-updates per second and the fraction of time the device had work.
 """
 import time
 from collections import deque
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
 from . import Batch, cases
 
 
+def _build(req):
+    _, nx, ny, steps, bumps = req
+    w, _ = cases.make_world(cases.member_params(nx, ny, steps), bumps)
+    return w
+
+
 class Server:
-    def __init__(self, capacity=64, hbm_budget=0.0, tick_steps=40, launch=None):
+    def __init__(self, capacity=64, hbm_budget=0.0, tick_steps=40, launch=None, builders=2, prefetch=8):
         self.batch = Batch(capacity=capacity, hbm_budget=hbm_budget)
         if launch:
             self.batch.set_launch(*launch)
         self.tick = tick_steps
+        self.pool = ThreadPoolExecutor(max_workers=builders)
+        self.prefetch = prefetch
 
     def close(self):
+        self.pool.shutdown(wait=True)
         self.batch.close()
 
     def run(self, requests, on_result=None, max_wall=None, log=None):
         """Serve `requests` = [(t_arrival_s, nx, ny, steps, bumps), ...] in
         real time; returns the statistics (and calls on_result(k, batch,
         member_id) before each run is retired, for callers that keep the
-        results)."""
+        results).  Waiting requests are admitted in arrival order; the
+        first `prefetch` of them have their worlds built ahead."""
         reqs = sorted(enumerate(requests), key=lambda r: r[1][0])
-        waiting = deque()
+        waiting = deque()  # [request index, request, future or None]
         live = {}  # member id -> (request index, arrival, admitted)
         done = {}
         cells_steps = 0.0
@@ -48,14 +62,16 @@ class Server:
             if max_wall and now > max_wall:
                 break
             while nxt < len(reqs) and reqs[nxt][1][0] <= now:
-                waiting.append(reqs[nxt])
+                waiting.append([reqs[nxt][0], reqs[nxt][1], None])
                 nxt += 1
-            while waiting:
-                k, (ta, nx, ny, steps, bumps) = waiting[0]
-                w, _ = cases.make_world(cases.member_params(nx, ny, steps), bumps)
+            for wq in list(waiting)[: self.prefetch]:
+                if wq[2] is None:
+                    wq[2] = self.pool.submit(_build, wq[1])
+            while waiting and (not live or waiting[0][2].done()):
+                k, (ta, nx, ny, steps, bumps), fut = waiting[0]
+                w = fut.result()
                 mid = self.batch.add(w, steps)
                 if mid is None:
-                    w.close()
                     break
                 waiting.popleft()
                 live[mid] = (k, ta, time.perf_counter() - t0)
@@ -80,7 +96,11 @@ class Server:
                 done[k] = {"arrival": ta, "admitted": tadm, "completed": tnow, "steps": st["steps"],
                            "failed_step": st["failed_step"]}
                 self.batch.remove(mid)
+        for wq in waiting:  # worlds built for requests never admitted (max_wall)
+            if wq[2] is not None:
+                wq[2].result().close()
         wall = time.perf_counter() - t0
+        self.records = done
         lat = np.array([d["completed"] - d["arrival"] for d in done.values()]) if done else np.array([np.nan])
         wait = np.array([d["admitted"] - d["arrival"] for d in done.values()]) if done else np.array([np.nan])
         return {

@@ -6,8 +6,9 @@ whatever the data), times whole steps for candidate launch shapes and keeps
 the fastest.  The knobs are the persistent workgroups of the height kernel
 and of the momentum kernel (multiples of the device's compute-unit count)
 and the steps per captured graph; the search is coordinate descent from the
-occupancy-derived default.  Results are cached per device and mix in a JSON
-file (default: tuned_launch.json beside this module, or $SW2_TUNE_CACHE).
+batch's default (the workgroups that stay resident at once).  Results are
+cached per device and mix in a JSON file (default: tuned_launch.json beside
+this module, or $SW2_TUNE_CACHE).
 """
 import json
 import os
@@ -15,17 +16,14 @@ import time
 from collections import Counter
 
 from . import Batch, cases
+from ._ffi import lib
 
 CACHE = os.environ.get("SW2_TUNE_CACHE", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_launch.json"))
 
 
 def device_cus():
-    """Compute units of the device (a new batch's default height-kernel grid
-    is three workgroups per compute unit)."""
-    b = Batch(capacity=1)
-    n = b.launch()[0] // 3
-    b.close()
-    return n
+    """Compute units of the device."""
+    return lib().sw2_device_cus()
 
 
 def mix_key(sizes, ncu):
@@ -65,7 +63,8 @@ def tune(sizes, steps=16, cache=CACHE, verbose=False):
         worlds.append(w)
         b.add(w, 1 << 30)
     t_start = time.perf_counter()
-    best = (3 * ncu, 4 * ncu, 8)
+    default = b.launch()  # the resident workgroups of each kernel, 8 steps per graph
+    best = default
     best_t = _time_steps(b, best, steps)
     tried = {best: best_t}
     for axis, options in ((0, [1, 2, 3, 4, 6]), (1, [1, 2, 3, 4, 6]), (2, [4, 8, 16])):
@@ -86,7 +85,7 @@ def tune(sizes, steps=16, cache=CACHE, verbose=False):
         w.close()
     rec = {"shape": list(best), "ms_per_step": best_t * 1e3, "candidates": len(tried),
            "tuning_s": time.perf_counter() - t_start,
-           "default_ms_per_step": tried[(3 * ncu, 4 * ncu, 8)] * 1e3}
+           "default": list(default), "default_ms_per_step": tried[default] * 1e3}
     db[key] = rec
     if cache:
         try:

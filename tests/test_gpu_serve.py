@@ -1,0 +1,56 @@
+"""The served stream: requests arriving on their own schedule join a running
+batch (continuous batching) and leave when done.  Each result must be the one
+the same request gives alone -- bit for bit, since a member runs the same
+arithmetic in the mix as alone -- and the runs must overlap on the device."""
+import numpy as np
+import pytest
+
+from sw2_amd import FORM_MIXED, Session, cases
+from sw2_amd.serve import Server
+
+pytestmark = pytest.mark.gpu
+
+
+def solo(nx, ny, steps, bumps):
+    w, _ = cases.make_world(cases.member_params(nx, ny, steps), bumps)
+    s = Session(w, FORM_MIXED)
+    s.begin(0, w.params.t0)
+    s.step(steps)
+    assert s.failed_step()[0] == 0
+    s.download()
+    out = {f: w.field(f).copy() for f in ("eta", "u", "v")}
+    s.close()
+    w.close()
+    return out
+
+
+def test_served_results_equal_solo_runs():
+    rng = np.random.default_rng(11)
+    sizes = [(128, 64), (256, 128), (192, 40), (512, 256)]
+    reqs = []
+    t = 0.0
+    for k in range(9):
+        t += float(rng.exponential(0.05))
+        nx, ny = sizes[k % len(sizes)]
+        reqs.append((t, nx, ny, int(rng.integers(30, 120)), cases.storm_bumps(rng)))
+    got = {}
+
+    def keep(k, batch, mid):
+        w = batch.download(mid)
+        got[k] = {f: w.field(f).copy() for f in ("eta", "u", "v")}
+
+    srv = Server(capacity=3, tick_steps=9)
+    st = srv.run(reqs, on_result=keep)
+    rec = srv.records
+    srv.close()
+    print(st, flush=True)
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert st["p50_s"] <= st["p99_s"]
+    # continuous batching: some request was admitted while another was still running
+    spans = sorted((r["admitted"], r["completed"]) for r in rec.values())
+    assert any(spans[i + 1][0] < spans[i][1] for i in range(len(spans) - 1))
+    for k, (_, nx, ny, steps, bumps) in enumerate(reqs):
+        assert rec[k]["steps"] == steps
+        ref = solo(nx, ny, steps, bumps)
+        for f in ("eta", "u", "v"):
+            assert np.array_equal(got[k][f], ref[f]), (k, f)
