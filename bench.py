@@ -20,7 +20,9 @@ Prints one JSON line:
                 measured HBM bytes per launch from the profiler summary in
                 profiles/ (byte counters scaled by a device copy), if present;
   cpu_baseline  the oracle (the CPU restatement of SW2's path) timed on a
-                bounded sample of the same workload on this host;
+                bounded sample of the same workload on this host, its row
+                sweeps on the host's CPUs (at most 16 threads), with a
+                single-thread sample beside it;
   mixed         the same grid in the mixed-precision form (FP32 anomalies
                 about an FP64 mean, 60 algorithmic bytes per cell-update) for
                 5000 steps, its speed-up over the FP64 form and its relative
@@ -55,7 +57,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=10, help="steps of the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the CPU baseline (0: the host's CPUs, at most 16)")
     ap.add_argument("--kernel-steps", type=int, default=100, help="steps timed per kernel after the timed region")
     ap.add_argument("--workload", default="c1", choices=["c0", "c1"])
     ap.add_argument("--copy-calib", action="store_true", help="only copy 3 x 512 MiB device to device and exit")
@@ -88,21 +91,31 @@ def copy_calib():
     print(json.dumps({"copy_calib_bytes": 3 * n}), flush=True)
 
 
-def cpu_baseline(p_fn, bumps, steps):
-    """The oracle on the same workload, `steps` steps, one core."""
+def cpu_baseline(p_fn, bumps, steps, threads):
+    """The oracle on the same workload: `steps` steps with its row sweeps on
+    `threads` threads (bit-equal to its serial run), and 2 steps on one
+    thread beside it."""
     from oracle import Oracle
     from sw2_amd import cases
     p = p_fn(steps)
     w, (eta, u, v) = cases.make_world(p, bumps)
-    o = Oracle(p)
-    o.load(eta, u, v)
-    w.close()
-    t0 = time.perf_counter()
-    o.step(steps)
-    dt = time.perf_counter() - t0
     cells = p.nx * p.ny
-    return {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{p.nx}x{p.ny}, {steps} steps of the same workload, single thread, {dt:.1f} s"}
+    out = {}
+    for th, n in ((threads, steps), (1, min(2, steps))):
+        o = Oracle(p)
+        o.load(eta, u, v)
+        o.set_threads(th)
+        t0 = time.perf_counter()
+        o.step(n)
+        out[th] = (n, time.perf_counter() - t0)
+        o.close()
+    w.close()
+    n, dt = out[threads]
+    n1, dt1 = out[1]
+    return {"value": cells * n / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{p.nx}x{p.ny}, {n} steps of the same workload, {threads} threads, {dt:.1f} s "
+                      f"(host: {os.cpu_count()} CPUs)",
+            "single_thread": {"value": cells * n1 / dt1, "cores": 1, "sample": f"{n1} steps, {dt1:.1f} s"}}
 
 
 def progress(msg):
@@ -270,9 +283,10 @@ def main():
     fused = s.fused
     s.close()
     w.close()
+    threads = a.cpu_threads if a.cpu_threads > 0 else max(1, min(16, len(os.sched_getaffinity(0))))
     if a.cpu_steps > 0:
-        progress(f"CPU baseline: {a.cpu_steps} oracle steps")
-    cpu = cpu_baseline(p_fn, bumps, a.cpu_steps) if a.cpu_steps > 0 else None
+        progress(f"CPU baseline: {a.cpu_steps} oracle steps on {threads} threads")
+    cpu = cpu_baseline(p_fn, bumps, a.cpu_steps, threads) if a.cpu_steps > 0 else None
     names = {"c1": "4096x2048 FP64 jet + storm, lat -85..85, Dt 10 s, del-4 5.97e10, minmod",
              "c0": "360x180 FP64 jet + storm, lat -80..80, Dt 60 s, del-4 1e15, minmod"}
     if kt:

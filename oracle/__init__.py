@@ -32,6 +32,8 @@ def lib():
         L.orc_rows.argtypes = [c_void_p, c_int]
         L.orc_rows.restype = POINTER(c_double)
         L.orc_size.argtypes = [c_void_p]
+        L.orc_set_threads.argtypes = [c_void_p, c_int]
+        L.orc_set_threads.restype = None
         L.orc_nsteps.argtypes = [c_void_p]
         L.orc_time.argtypes = [c_void_p]
         L.orc_time.restype = c_double
@@ -75,6 +77,11 @@ class Oracle:
     def rows(self, name):
         p = lib().orc_rows(self.h, ROW_ID[name])
         return np.ctypeslib.as_array(p, shape=(self.ny + 4,)).copy()
+
+    def set_threads(self, n):
+        """Sweep rows on n threads (same results bit for bit; the timed CPU
+        baseline uses this)."""
+        lib().orc_set_threads(self.h, int(n))
 
     def preloop(self):
         if lib().orc_preloop(self.h):

@@ -55,11 +55,23 @@ typedef struct orc {
     int prn;
     double last_dtcfl;
     int tracer_active;
+    int threads;
 } orc;
 
 #define IX(i, j) ((i) + 1 + ((j) + 1) * o->sx)
 #define RJ(j) ((j) + 1)
 #define F(a, i, j) (a)[IX(i, j)]
+
+/* Rows of a sweep are independent wherever ROWS stands: each iteration
+ * writes its own row of the output from inputs no iteration writes, so the
+ * threaded sweep gives the serial result bit for bit.  Threads are used only
+ * when orc_set_threads asks for them (the timed CPU baseline); reductions
+ * stay serial or combine order-free maxima and minima. */
+#ifdef _OPENMP
+#define ROWS _Pragma("omp parallel for schedule(static) if (o->threads > 1) num_threads(o->threads)")
+#else
+#define ROWS
+#endif
 
 /* ---------------- geometry (sw.h:113-204, coords.c:36-59, tools.c:256-273) */
 static double lin(int i, double x0, double x1, int n) {
@@ -229,6 +241,7 @@ static inline double r2rM(double Bcurr, double Bpost, double Bnext) {
     double flux = CwP * BwP + CwM * BwM - CeP * BeP - CeM * BeM + CsP * BsP + CsM * BsM - CnP * BnP - CnM * BnM;
 
 static inline __attribute__((always_inline)) void body_h(orc *o, psifn G, double *out, double Dt) {
+    ROWS
     for (int j = 1; j <= o->ny; j++)
         for (int i = 1; i <= o->nx; i++) {
             FLUXBODY(G, HH, i, j, Dt)
@@ -239,6 +252,7 @@ static inline __attribute__((always_inline)) void body_h(orc *o, psifn G, double
 }
 
 static inline __attribute__((always_inline)) void body_tracer(orc *o, psifn G, double *out, double Dt) {
+    ROWS
     for (int j = 1; j <= o->ny; j++)
         for (int i = 1; i <= o->nx; i++) {
             FLUXBODY(G, TR, i, j, Dt)
@@ -252,6 +266,7 @@ static inline __attribute__((always_inline)) void body_tracer(orc *o, psifn G, d
 
 /* BODYU (calc.c:168-200) */
 static inline __attribute__((always_inline)) void body_u(orc *o, psifn G, double *out, double Dt) {
+    ROWS
     for (int j = 1; j <= o->ny; j++) {
         double dxc = o->dxc[RJ(j)], dyv = o->dyv[RJ(j)], dxe = o->dxe[RJ(j)];
         for (int i = 1; i <= o->nx; i++) {
@@ -289,6 +304,7 @@ static inline __attribute__((always_inline)) void body_u(orc *o, psifn G, double
 
 /* BODYV (calc.c:205-237) */
 static inline __attribute__((always_inline)) void body_v(orc *o, psifn G, double *out, double Dt) {
+    ROWS
     for (int j = 1; j <= o->ny; j++) {
         double dxv = o->dxv[RJ(j)], dyc = o->dyc[RJ(j)];
         double dyn2 = o->dyn[RJ(j)] + o->dyn[RJ(j + 1)];
@@ -345,6 +361,7 @@ static void laplacian_vels_o(orc *o, double *uin, double *vin, double *uout, dou
     double *ze = o->ze, *di = o->di;
     memset(ze, 0, sizeof(double) * o->S);
     memset(di, 0, sizeof(double) * o->S);
+    ROWS
     for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx; i++) {
             double a = (1.0 / o->dxv[RJ(j)]), b = -(1.0 / o->dyc[RJ(j + 1)]);
@@ -357,6 +374,7 @@ static void laplacian_vels_o(orc *o, double *uin, double *vin, double *uout, dou
     halo_update_o(o, di);
     for (int i = 1; i <= nx; i++) F(di, i, ny + 1) = (F(uin, i, ny) - F(uin, i - 1, ny)) / o->dxe[RJ(ny)];
     for (int i = 1; i <= nx; i++) F(ze, i, 0) = 0;
+    ROWS
     for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx; i++) {
             double a = -(1.0 / o->dyn[RJ(j)]), b = (1.0 / o->dxc[RJ(j)]);
@@ -383,6 +401,7 @@ static void laplacian_eta_o(orc *o, double *ein, double *eout) {
      * second loop reads (psi(0,j), xi(i,0)) are never written: zero here */
     memset(o->psi, 0, sizeof(double) * o->S);
     memset(o->xi, 0, sizeof(double) * o->S);
+    ROWS
     for (int j = 1; j <= ny; j++) {
         double latc = LATC(j), latn = LATN(j);
         double fpsi = RM(latc) / RZ(latc), fxi = RZ(latn) / RM(latn);
@@ -391,6 +410,7 @@ static void laplacian_eta_o(orc *o, double *ein, double *eout) {
             F(o->xi, i, j) = fxi * (F(ein, i, j + 1) - F(ein, i, j)) / (o->dyc[RJ(j)]);
         }
     }
+    ROWS
     for (int j = 1; j <= ny; j++) {
         double latc = LATC(j);
         double k = 1.0 / (RM(latc) * RZ(latc));
@@ -407,6 +427,7 @@ static void laplacian_eta_o(orc *o, double *ein, double *eout) {
 
 /* ---------------- AB, pressure, Coriolis, CFL, stats -------------------- */
 static void adams_bashforth(orc *o, double *nab, double *d, double *m1, double *m2, int k) {
+    ROWS
     for (int j = 1; j <= o->ny; j++)
         for (int i = 1; i <= o->nx; i++) {
             if (k == 0)
@@ -422,6 +443,9 @@ static void adams_bashforth(orc *o, double *nab, double *d, double *m1, double *
 
 static double compute_dt_o(orc *o) {
     double max_x = 0, max_y = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) reduction(max : max_x, max_y) if (o->threads > 1) num_threads(o->threads)
+#endif
     for (int j = 1; j <= o->ny; j++)
         for (int i = 1; i <= o->nx; i++) {
             if (fabs(UW(i, j) / o->dxe[RJ(j)]) > max_x) max_x = fabs(UW(i, j)) / o->dxe[RJ(j)];
@@ -440,6 +464,31 @@ static double compute_dt_o(orc *o) {
 /* get_stats_scaf + check_stats_scaf (sppde_core.c:346-395) */
 static int check_stats_o(orc *o, double *r, const char *name, double *smin, double *smax, double *savg) {
     double s = 0, n = 0, mn = 0, mx = 0;
+#ifdef _OPENMP
+    if (o->threads > 1 && !savg) {
+        /* the serial scan seeds min and max with cell (1,1) and then only
+         * lets comparisons replace them: a NaN there sticks, a NaN elsewhere
+         * is never taken.  Seeding every thread with cell (1,1) and merging
+         * by the same comparisons gives the same min and max. */
+        mn = mx = F(r, 1, 1);
+#pragma omp parallel num_threads(o->threads)
+        {
+            double lmn = mn, lmx = mx;
+#pragma omp for schedule(static)
+            for (int j = 1; j <= o->ny; j++)
+                for (int i = 1; i <= o->nx; i++) {
+                    if (F(r, i, j) < lmn) lmn = F(r, i, j);
+                    if (F(r, i, j) > lmx) lmx = F(r, i, j);
+                }
+#pragma omp critical
+            {
+                if (lmn < mn) mn = lmn;
+                if (lmx > mx) mx = lmx;
+            }
+        }
+        n = (double)o->nx * o->ny;
+    } else
+#endif
     for (int j = 1; j <= o->ny; j++)
         for (int i = 1; i <= o->nx; i++) {
             n = n + 1;
@@ -699,6 +748,7 @@ double *orc_rows(orc *o, int which) {
     return (which >= 0 && which < 14) ? f[which] : NULL;
 }
 int orc_size(orc *o) { return o->S; }
+void orc_set_threads(orc *o, int n) { o->threads = n > 1 ? n : 1; }
 int orc_nsteps(orc *o) { return o->N; }
 double orc_time(orc *o) { return o->t; }
 int orc_step_index(orc *o) { return o->n; }
@@ -794,6 +844,7 @@ static int step_once(orc *o) {
     DISPATCH(body_tracer, o->Dtr[0], Dt)
     adams_bashforth(o, o->Trab, o->Dtr[0], o->Dtr[1], o->Dtr[2], k);
     rotate(o->Dtr);
+    ROWS
     for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx; i++) F(o->tracer, i, j) = F(o->tracer, i, j) + F(o->Trab, i, j);
     halo_update_o(o, o->tracer);
@@ -808,6 +859,7 @@ static int step_once(orc *o) {
             laplacian_eta_o(o, o->le, o->lle);
             if (o->nu6 != 0) laplacian_eta_o(o, o->lle, o->llle);
         }
+        ROWS
         for (int j = 1; j <= ny; j++)
             for (int i = 1; i <= nx; i++) {
                 F(o->Deta[0], i, j) += (o->nu2) * (o->Dt) * F(o->le, i, j);
@@ -816,28 +868,33 @@ static int step_once(orc *o) {
             }
     }
     adams_bashforth(o, o->Etab, o->Deta[0], o->Deta[1], o->Deta[2], k);
+    ROWS
     for (int j = 1; j <= ny; j++) {
         double gf = Dt * sin(LATC(j)) / RZ(LATC(j));
         for (int i = 1; i <= nx; i++) F(o->Etab, i, j) -= (gf * HH(i, j) * 0.5 * (VW(i, j) + VW(i, j - 1)));
     }
     rotate(o->Deta);
+    ROWS
     for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx; i++) F(o->eta, i, j) = F(o->eta, i, j) + F(o->Etab, i, j);
     halo_update_o(o, o->eta);
     /* velocities (core.c:212-280) */
     DISPATCH(body_u, o->Duadv, Dt)
     DISPATCH(body_v, o->Dvadv, Dt)
+    ROWS
     for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx; i++) {
             F(o->Dupres, i, j) = -o->ge[RJ(j)] * Dt * (F(o->eta, i + 1, j) - F(o->eta, i, j)) / o->dxc[RJ(j)];
             F(o->Dvpres, i, j) = -o->gn[RJ(j)] * Dt * (F(o->eta, i, j + 1) - F(o->eta, i, j)) / o->dyc[RJ(j + 1)];
         }
     double *Dun = o->Du[0], *Dvn = o->Dv[0];
+    ROWS
     for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx; i++) {
             F(Dun, i, j) = F(o->Duadv, i, j) + F(o->Dupres, i, j);
             F(Dvn, i, j) = F(o->Dvadv, i, j) + F(o->Dvpres, i, j);
         }
+    ROWS
     for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx; i++) { /* dissipation_effect, calc.c:396-397 */
             F(Dun, i, j) -= F(o->u, i, j) * o->sigma_dis * o->Dt;
@@ -851,6 +908,7 @@ static int step_once(orc *o) {
             laplacian_vels_o(o, o->Lu, o->Lv, o->LLu, o->LLv);
             if (o->nu6 != 0) laplacian_vels_o(o, o->LLu, o->LLv, o->LLLu, o->LLLv);
         }
+        ROWS
         for (int j = 1; j <= ny; j++)
             for (int i = 1; i <= nx; i++) {
                 F(Dun, i, j) += (o->nu2) * (o->Dt) * F(o->Lu, i, j);
@@ -866,6 +924,7 @@ static int step_once(orc *o) {
     rotate(o->Du);
     rotate(o->Dv);
     /* Coriolis, u then v, in place (calc.c:542-598) */
+    ROWS
     for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx; i++) {
             double semiU = 0.25 * (F(o->v, i, j) + F(o->v, i + 1, j) + F(o->v, i, j - 1) + F(o->v, i + 1, j - 1));
@@ -873,6 +932,7 @@ static int step_once(orc *o) {
             double Beta = 0.25 * Alpha * Alpha;
             F(o->u, i, j) = (F(o->u, i, j) + F(o->Duab, i, j) - Beta * F(o->u, i, j) + Alpha * semiU) / (1.0 + Beta);
         }
+    ROWS
     for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx; i++) {
             double semiV = 0.25 * (F(o->u, i - 1, j + 1) + F(o->u, i, j + 1) + F(o->u, i, j) + F(o->u, i - 1, j));

@@ -941,7 +941,7 @@ static void launch_fused(sw2_session *s, int n, int k, int slot) {
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 1], s->st);
     UVArgs ua{s->eta2, s->u, s->v, s->Du[1], s->Du[2], s->Dv[1], s->Dv[2], s->Dtr[0], s->Dtr[1], s->Dtr[2],
               s->u2, s->v2, s->Du[0], s->Dv[0], s->tr, n, k, slot ^ 1, s->Dt, s->sigma, s->c2, s->c4};
-    kf_uv<SCH, HV><<<G, NTU, 0, s->st>>>(s->g, s->fl, ua);
+    kf_uv<SCH, HV><<<G, NTF, 0, s->st>>>(s->g, s->fl, ua);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 2], s->st);
 }
 

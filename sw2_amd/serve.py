@@ -43,6 +43,12 @@ class Server:
         self.pool.shutdown(wait=True)
         self.batch.close()
 
+    def _prefetch(self, waiting):
+        """Start building the worlds of the first `prefetch` waiting requests."""
+        for wq in list(waiting)[: self.prefetch]:
+            if wq[2] is None:
+                wq[2] = self.pool.submit(_build, wq[1])
+
     def run(self, requests, on_result=None, max_wall=None, log=None):
         """Serve `requests` = [(t_arrival_s, nx, ny, steps, bumps), ...] in
         real time; returns the statistics (and calls on_result(k, batch,
@@ -53,6 +59,7 @@ class Server:
         waiting = deque()  # [request index, request, future or None]
         live = {}  # member id -> (request index, arrival, admitted)
         done = {}
+        rejected = []
         cells_steps = 0.0
         busy = 0.0
         t0 = time.perf_counter()
@@ -64,18 +71,28 @@ class Server:
             while nxt < len(reqs) and reqs[nxt][1][0] <= now:
                 waiting.append([reqs[nxt][0], reqs[nxt][1], None])
                 nxt += 1
-            for wq in list(waiting)[: self.prefetch]:
-                if wq[2] is None:
-                    wq[2] = self.pool.submit(_build, wq[1])
-            while waiting and (not live or waiting[0][2].done()):
+            self._prefetch(waiting)
+            # admit in arrival order; with runs on the device only worlds
+            # already built are taken, with none the head is waited for
+            while waiting:
+                self._prefetch(waiting)
                 k, (ta, nx, ny, steps, bumps), fut = waiting[0]
+                if live and not fut.done():
+                    break
                 w = fut.result()
                 mid = self.batch.add(w, steps)
+                if mid is None and not live:
+                    # no room even in an empty batch: it can never run here
+                    waiting.popleft()
+                    w.close()
+                    rejected.append(k)
+                    continue
                 if mid is None:
                     break
                 waiting.popleft()
                 live[mid] = (k, ta, time.perf_counter() - t0)
                 cells_steps += float(nx) * ny * steps
+            self._prefetch(waiting)
             if not live:
                 if nxt < len(reqs):
                     time.sleep(max(0.0, min(0.05, reqs[nxt][1][0] - (time.perf_counter() - t0))))
@@ -104,7 +121,7 @@ class Server:
         lat = np.array([d["completed"] - d["arrival"] for d in done.values()]) if done else np.array([np.nan])
         wait = np.array([d["admitted"] - d["arrival"] for d in done.values()]) if done else np.array([np.nan])
         return {
-            "requests": len(requests), "completed": len(done),
+            "requests": len(requests), "completed": len(done), "rejected": len(rejected),
             "failed": sum(1 for d in done.values() if d["failed_step"]),
             "wall_s": wall, "runs_per_hour": len(done) / wall * 3600.0,
             "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),

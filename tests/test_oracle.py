@@ -289,3 +289,29 @@ def test_guards_fire():
     w, (eta, u, v) = cases.make_world(p, cases.single_bump(1))
     o = loaded(p, eta, u, v)
     assert o.step(3)[0] == 1 and "Dt_CFL" in o.failmsg()
+
+
+def test_threaded_sweeps_equal_serial():
+    """The timed CPU baseline sweeps rows on several threads: every field,
+    the CFL value and the guards must be the serial run's, bit for bit."""
+    p = cases.scaled_params(160, 64, steps=40, lat=75.0)
+    w, (eta, u, v) = cases.make_world(p, cases.single_bump(4))
+    runs = []
+    for threads in (1, 4):
+        o = loaded(p, eta, u, v)
+        o.set_threads(threads)
+        assert o.step(40)[0] == 0
+        runs.append(o)
+    for f in ("eta", "u", "v", "tracer"):
+        assert np.array_equal(runs[0].field(f), runs[1].field(f)), f
+    assert runs[0].last_dtcfl() == runs[1].last_dtcfl()
+    # the guards: a range failure and a NaN at cell (1,1) are caught alike
+    for bad in (5e10, np.nan):
+        e = eta.copy()
+        e[0, 0] = bad
+        fails = []
+        for threads in (1, 4):
+            o = loaded(p, e, u, v)
+            o.set_threads(threads)
+            fails.append((o.step(3)[0], o.failmsg()))
+        assert fails[0] == fails[1] and fails[0][0] > 0, fails
