@@ -203,6 +203,28 @@ void halo_update(double *x, map *M);
 void do_channel(sw *SW);
 void calc_energy_balance(double *ke, double *ape, sw *SW);
 
+/* SW2's per-step functions (sw.h:229-297), run on the GPU.  Each reads SW's
+ * host arrays as the reference function does and writes back the cells it
+ * writes: the interior of the output for the tendencies, the whole arrays
+ * for the Laplacians, u or v in place for Coriolis.  One world at a time
+ * (its device copy is cached between calls); not re-entrant; errors CRASH.
+ * Results are SW2's bit for bit. */
+void solve_tracer(double *Dtracer_n, double Dt, sw *SW);                  /* calc.c:241-275 */
+void solve_cons_h(double *Deta_n, double Dt, sw *SW);                     /* calc.c:289-355 */
+void dissipation_effect(double *Du_n, double *Dv_n, sw *SW);              /* calc.c:357-400 */
+void do_eta_geofactor(double *Deta_nAB, double Dt, sw *SW);               /* calc.c:403-422 */
+void solve_adv_u(double *Du_adv, double Dt, sw *SW);                      /* calc.c:424-460 */
+void solve_adv_v(double *Dv_adv, double Dt, sw *SW);                      /* calc.c:462-498 */
+void solve_pres_u(double *Du_pres, double Dt, sw *SW);                    /* calc.c:500-519 */
+void solve_pres_v(double *Dv_pres, double Dt, sw *SW);                    /* calc.c:521-540 */
+void solve_Coriolis_u(double *Du_nAB, double Dt, sw *SW);                 /* calc.c:542-569 */
+void solve_Coriolis_v(double *Dv_nAB, double Dt, sw *SW);                 /* calc.c:571-598 */
+void do_AdamsBashforth(double *Dvar_nAB, double *Dvar_n, double *Dvar_nM1, double *Dvar_nM2, int n,
+                       map *M);                                           /* calc.c:651-699 */
+double compute_Dt(double Courant, sw *SW);                                /* calc.c:701-738 */
+void laplacian_vels(double *u_in, double *v_in, double *u_out, double *v_out, sw *SW); /* laplacian.c:51-111 */
+void laplacian_eta(double *e_in, double *e_out, sw *SW);                  /* laplacian.c:114-151 */
+
 /* Mass diagnostic (no counterpart in SW2): sum of H*Dxc*Dyc over the
  * interior in SW2's loop order. */
 double sw2_mass(sw *SW);
@@ -294,6 +316,9 @@ int sw2_batch_get_launch(sw2_batch *b, int *blocks_eta, int *blocks_uv, int *ste
  * step); returns without waiting. */
 int sw2_batch_step(sw2_batch *b, int nsteps);
 int sw2_batch_sync(sw2_batch *b);
+/* Hold a member (paused != 0: its tiles are skipped and its step count
+ * stays) or release it, between steps; a scheduler's knob. */
+int sw2_batch_pause(sw2_batch *b, int id, int paused);
 /* Steps completed, whether the member's run ended, first failed step. */
 int sw2_batch_member(sw2_batch *b, int id, int *steps_done, int *done, int *failed_step);
 /* Ids of members whose runs ended; returns how many (<= maxids). */

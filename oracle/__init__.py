@@ -34,6 +34,20 @@ def lib():
         L.orc_size.argtypes = [c_void_p]
         L.orc_set_threads.argtypes = [c_void_p, c_int]
         L.orc_set_threads.restype = None
+        for name in ("orc_solve_tracer", "orc_solve_cons_h", "orc_do_eta_geofactor", "orc_solve_adv_u", "orc_solve_adv_v",
+                     "orc_solve_pres_u", "orc_solve_pres_v", "orc_solve_coriolis_u", "orc_solve_coriolis_v"):
+            getattr(L, name).argtypes = [c_void_p, POINTER(c_double), c_double]
+            getattr(L, name).restype = None
+        L.orc_dissipation_effect.argtypes = [c_void_p, POINTER(c_double), POINTER(c_double)]
+        L.orc_dissipation_effect.restype = None
+        L.orc_adams_bashforth.argtypes = [c_void_p] + [POINTER(c_double)] * 4 + [c_int]
+        L.orc_adams_bashforth.restype = None
+        L.orc_compute_dt.argtypes = [c_void_p, c_double]
+        L.orc_compute_dt.restype = c_double
+        L.orc_laplacian_vels.argtypes = [c_void_p] + [POINTER(c_double)] * 4
+        L.orc_laplacian_vels.restype = None
+        L.orc_laplacian_eta.argtypes = [c_void_p] + [POINTER(c_double)] * 2
+        L.orc_laplacian_eta.restype = None
         L.orc_nsteps.argtypes = [c_void_p]
         L.orc_time.argtypes = [c_void_p]
         L.orc_time.restype = c_double
@@ -82,6 +96,12 @@ class Oracle:
         """Sweep rows on n threads (same results bit for bit; the timed CPU
         baseline uses this)."""
         lib().orc_set_threads(self.h, int(n))
+
+    def call(self, name, *args):
+        """One of SW2's per-step functions on the oracle's fields
+        (orc_<name>); numpy arrays are passed as pointers."""
+        conv = [a.ctypes.data_as(POINTER(c_double)) if isinstance(a, np.ndarray) else a for a in args]
+        return getattr(lib(), "orc_" + name)(self.h, *conv)
 
     def preloop(self):
         if lib().orc_preloop(self.h):

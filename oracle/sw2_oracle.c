@@ -241,7 +241,7 @@ static inline double r2rM(double Bcurr, double Bpost, double Bnext) {
     double flux = CwP * BwP + CwM * BwM - CeP * BeP - CeM * BeM + CsP * BsP + CsM * BsM - CnP * BnP - CnM * BnM;
 
 static inline __attribute__((always_inline)) void body_h(orc *o, psifn G, double *out, double Dt) {
-    ROWS
+ROWS
     for (int j = 1; j <= o->ny; j++)
         for (int i = 1; i <= o->nx; i++) {
             FLUXBODY(G, HH, i, j, Dt)
@@ -252,7 +252,7 @@ static inline __attribute__((always_inline)) void body_h(orc *o, psifn G, double
 }
 
 static inline __attribute__((always_inline)) void body_tracer(orc *o, psifn G, double *out, double Dt) {
-    ROWS
+ROWS
     for (int j = 1; j <= o->ny; j++)
         for (int i = 1; i <= o->nx; i++) {
             FLUXBODY(G, TR, i, j, Dt)
@@ -266,7 +266,7 @@ static inline __attribute__((always_inline)) void body_tracer(orc *o, psifn G, d
 
 /* BODYU (calc.c:168-200) */
 static inline __attribute__((always_inline)) void body_u(orc *o, psifn G, double *out, double Dt) {
-    ROWS
+ROWS
     for (int j = 1; j <= o->ny; j++) {
         double dxc = o->dxc[RJ(j)], dyv = o->dyv[RJ(j)], dxe = o->dxe[RJ(j)];
         for (int i = 1; i <= o->nx; i++) {
@@ -304,7 +304,7 @@ static inline __attribute__((always_inline)) void body_u(orc *o, psifn G, double
 
 /* BODYV (calc.c:205-237) */
 static inline __attribute__((always_inline)) void body_v(orc *o, psifn G, double *out, double Dt) {
-    ROWS
+ROWS
     for (int j = 1; j <= o->ny; j++) {
         double dxv = o->dxv[RJ(j)], dyc = o->dyc[RJ(j)];
         double dyn2 = o->dyn[RJ(j)] + o->dyn[RJ(j + 1)];
@@ -361,7 +361,7 @@ static void laplacian_vels_o(orc *o, double *uin, double *vin, double *uout, dou
     double *ze = o->ze, *di = o->di;
     memset(ze, 0, sizeof(double) * o->S);
     memset(di, 0, sizeof(double) * o->S);
-    ROWS
+ROWS
     for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx; i++) {
             double a = (1.0 / o->dxv[RJ(j)]), b = -(1.0 / o->dyc[RJ(j + 1)]);
@@ -374,7 +374,7 @@ static void laplacian_vels_o(orc *o, double *uin, double *vin, double *uout, dou
     halo_update_o(o, di);
     for (int i = 1; i <= nx; i++) F(di, i, ny + 1) = (F(uin, i, ny) - F(uin, i - 1, ny)) / o->dxe[RJ(ny)];
     for (int i = 1; i <= nx; i++) F(ze, i, 0) = 0;
-    ROWS
+ROWS
     for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx; i++) {
             double a = -(1.0 / o->dyn[RJ(j)]), b = (1.0 / o->dxc[RJ(j)]);
@@ -401,7 +401,7 @@ static void laplacian_eta_o(orc *o, double *ein, double *eout) {
      * second loop reads (psi(0,j), xi(i,0)) are never written: zero here */
     memset(o->psi, 0, sizeof(double) * o->S);
     memset(o->xi, 0, sizeof(double) * o->S);
-    ROWS
+ROWS
     for (int j = 1; j <= ny; j++) {
         double latc = LATC(j), latn = LATN(j);
         double fpsi = RM(latc) / RZ(latc), fxi = RZ(latn) / RM(latn);
@@ -410,7 +410,7 @@ static void laplacian_eta_o(orc *o, double *ein, double *eout) {
             F(o->xi, i, j) = fxi * (F(ein, i, j + 1) - F(ein, i, j)) / (o->dyc[RJ(j)]);
         }
     }
-    ROWS
+ROWS
     for (int j = 1; j <= ny; j++) {
         double latc = LATC(j);
         double k = 1.0 / (RM(latc) * RZ(latc));
@@ -427,7 +427,7 @@ static void laplacian_eta_o(orc *o, double *ein, double *eout) {
 
 /* ---------------- AB, pressure, Coriolis, CFL, stats -------------------- */
 static void adams_bashforth(orc *o, double *nab, double *d, double *m1, double *m2, int k) {
-    ROWS
+ROWS
     for (int j = 1; j <= o->ny; j++)
         for (int i = 1; i <= o->nx; i++) {
             if (k == 0)
@@ -441,7 +441,7 @@ static void adams_bashforth(orc *o, double *nab, double *d, double *m1, double *
         }
 }
 
-static double compute_dt_o(orc *o) {
+static double compute_dt_c(orc *o, double Courant) {
     double max_x = 0, max_y = 0;
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static) reduction(max : max_x, max_y) if (o->threads > 1) num_threads(o->threads)
@@ -458,8 +458,9 @@ static double compute_dt_o(orc *o) {
                 if (fabs(sqrt((o->depth) * (o->gn[RJ(j)])) / o->dyn[RJ(j)]) > max_y) max_y = fabs(sqrt((o->depth) * (o->gn[RJ(j)]))) / o->dyn[RJ(j)];
             }
     }
-    return (o->Courant / (max_x + max_y));
+    return (Courant / (max_x + max_y));
 }
+static double compute_dt_o(orc *o) { return compute_dt_c(o, o->Courant); }
 
 /* get_stats_scaf + check_stats_scaf (sppde_core.c:346-395) */
 static int check_stats_o(orc *o, double *r, const char *name, double *smin, double *smax, double *savg) {
@@ -823,6 +824,91 @@ static void rotate(double *h[3]) {
     h[0] = m2;
 }
 
+/* solve_cons_h (calc.c:289-355): BODYH, then the hyperviscosity of
+ * laplacian_eta applied to u (calc.c:332) */
+static void cons_h(orc *o, double *out, double Dt) {
+    int nx = o->nx, ny = o->ny;
+    DISPATCH(body_h, out, Dt)
+    if (o->nu2 != 0 || o->nu4 != 0 || o->nu6 != 0) {
+        memset(o->le, 0, sizeof(double) * o->S);
+        memset(o->lle, 0, sizeof(double) * o->S);
+        memset(o->llle, 0, sizeof(double) * o->S);
+        laplacian_eta_o(o, o->u, o->le); /* SW2 passes u here (calc.c:332) */
+        if (o->nu4 != 0 || o->nu6 != 0) {
+            laplacian_eta_o(o, o->le, o->lle);
+            if (o->nu6 != 0) laplacian_eta_o(o, o->lle, o->llle);
+        }
+ROWS
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                F(out, i, j) += (o->nu2) * (o->Dt) * F(o->le, i, j);
+                F(out, i, j) += (o->nu4) * (o->Dt) * F(o->lle, i, j);
+                F(out, i, j) += (o->nu6) * (o->Dt) * F(o->llle, i, j);
+            }
+    }
+}
+
+/* do_eta_geofactor (calc.c:403-422) */
+static void geofactor(orc *o, double *d, double Dt) {
+    int nx = o->nx, ny = o->ny;
+ROWS
+    for (int j = 1; j <= ny; j++) {
+        double gf = Dt * sin(LATC(j)) / RZ(LATC(j));
+        for (int i = 1; i <= nx; i++) F(d, i, j) -= (gf * HH(i, j) * 0.5 * (VW(i, j) + VW(i, j - 1)));
+    }
+}
+
+/* solve_pres_u / solve_pres_v (calc.c:500-540) */
+static void pres_u(orc *o, double *out, double Dt) {
+    int nx = o->nx, ny = o->ny;
+ROWS
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) F(out, i, j) = -o->ge[RJ(j)] * Dt * (F(o->eta, i + 1, j) - F(o->eta, i, j)) / o->dxc[RJ(j)];
+}
+static void pres_v(orc *o, double *out, double Dt) {
+    int nx = o->nx, ny = o->ny;
+ROWS
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) F(out, i, j) = -o->gn[RJ(j)] * Dt * (F(o->eta, i, j + 1) - F(o->eta, i, j)) / o->dyc[RJ(j + 1)];
+}
+
+/* dissipation_effect (calc.c:357-400), channel worlds */
+static void dissipation(orc *o, double *Dun, double *Dvn) {
+    int nx = o->nx, ny = o->ny;
+ROWS
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) { /* calc.c:396-397 */
+            F(Dun, i, j) -= F(o->u, i, j) * o->sigma_dis * o->Dt;
+            F(Dvn, i, j) -= F(o->v, i, j) * o->sigma_dis * o->Dt;
+        }
+}
+
+/* solve_Coriolis_u (calc.c:542-569), in place on u */
+static void coriolis_u(orc *o, const double *Duab, double Dt) {
+    int nx = o->nx, ny = o->ny;
+ROWS
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            double semiU = 0.25 * (F(o->v, i, j) + F(o->v, i + 1, j) + F(o->v, i, j - 1) + F(o->v, i + 1, j - 1));
+            double Alpha = Dt * o->fe[RJ(j)] + (Dt * F(o->u, i, j) * sin(LATC(j))) / RZ(LATC(j));
+            double Beta = 0.25 * Alpha * Alpha;
+            F(o->u, i, j) = (F(o->u, i, j) + F(Duab, i, j) - Beta * F(o->u, i, j) + Alpha * semiU) / (1.0 + Beta);
+        }
+}
+
+/* solve_Coriolis_v (calc.c:571-598), in place on v, reading u as it stands */
+static void coriolis_v(orc *o, const double *Dvab, double Dt) {
+    int nx = o->nx, ny = o->ny;
+ROWS
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            double semiV = 0.25 * (F(o->u, i - 1, j + 1) + F(o->u, i, j + 1) + F(o->u, i, j) + F(o->u, i - 1, j));
+            double Alpha = Dt * o->fn[RJ(j)] + (Dt * semiV * sin(LATN(j))) / RZ(LATN(j));
+            double Beta = 0.25 * Alpha * Alpha;
+            F(o->v, i, j) = (F(o->v, i, j) + F(Dvab, i, j) - Beta * F(o->v, i, j) - Alpha * semiV) / (1.0 + Beta);
+        }
+}
+
 /* One step of loop() (core.c:134-338) without IO.  Returns 0, or 1 when SW2
  * would CRASH at this step (message in orc_failmsg). */
 static int step_once(orc *o) {
@@ -844,62 +930,32 @@ static int step_once(orc *o) {
     DISPATCH(body_tracer, o->Dtr[0], Dt)
     adams_bashforth(o, o->Trab, o->Dtr[0], o->Dtr[1], o->Dtr[2], k);
     rotate(o->Dtr);
-    ROWS
+ROWS
     for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx; i++) F(o->tracer, i, j) = F(o->tracer, i, j) + F(o->Trab, i, j);
     halo_update_o(o, o->tracer);
     /* eta (core.c:184-210; calc.c:289-355, 403-422) */
-    DISPATCH(body_h, o->Deta[0], Dt)
-    if (o->nu2 != 0 || o->nu4 != 0 || o->nu6 != 0) {
-        memset(o->le, 0, sizeof(double) * o->S);
-        memset(o->lle, 0, sizeof(double) * o->S);
-        memset(o->llle, 0, sizeof(double) * o->S);
-        laplacian_eta_o(o, o->u, o->le); /* SW2 passes u here (calc.c:332) */
-        if (o->nu4 != 0 || o->nu6 != 0) {
-            laplacian_eta_o(o, o->le, o->lle);
-            if (o->nu6 != 0) laplacian_eta_o(o, o->lle, o->llle);
-        }
-        ROWS
-        for (int j = 1; j <= ny; j++)
-            for (int i = 1; i <= nx; i++) {
-                F(o->Deta[0], i, j) += (o->nu2) * (o->Dt) * F(o->le, i, j);
-                F(o->Deta[0], i, j) += (o->nu4) * (o->Dt) * F(o->lle, i, j);
-                F(o->Deta[0], i, j) += (o->nu6) * (o->Dt) * F(o->llle, i, j);
-            }
-    }
+    cons_h(o, o->Deta[0], Dt);
     adams_bashforth(o, o->Etab, o->Deta[0], o->Deta[1], o->Deta[2], k);
-    ROWS
-    for (int j = 1; j <= ny; j++) {
-        double gf = Dt * sin(LATC(j)) / RZ(LATC(j));
-        for (int i = 1; i <= nx; i++) F(o->Etab, i, j) -= (gf * HH(i, j) * 0.5 * (VW(i, j) + VW(i, j - 1)));
-    }
+    geofactor(o, o->Etab, Dt);
     rotate(o->Deta);
-    ROWS
+ROWS
     for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx; i++) F(o->eta, i, j) = F(o->eta, i, j) + F(o->Etab, i, j);
     halo_update_o(o, o->eta);
     /* velocities (core.c:212-280) */
     DISPATCH(body_u, o->Duadv, Dt)
     DISPATCH(body_v, o->Dvadv, Dt)
-    ROWS
-    for (int j = 1; j <= ny; j++)
-        for (int i = 1; i <= nx; i++) {
-            F(o->Dupres, i, j) = -o->ge[RJ(j)] * Dt * (F(o->eta, i + 1, j) - F(o->eta, i, j)) / o->dxc[RJ(j)];
-            F(o->Dvpres, i, j) = -o->gn[RJ(j)] * Dt * (F(o->eta, i, j + 1) - F(o->eta, i, j)) / o->dyc[RJ(j + 1)];
-        }
+    pres_u(o, o->Dupres, Dt);
+    pres_v(o, o->Dvpres, Dt);
     double *Dun = o->Du[0], *Dvn = o->Dv[0];
-    ROWS
+ROWS
     for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx; i++) {
             F(Dun, i, j) = F(o->Duadv, i, j) + F(o->Dupres, i, j);
             F(Dvn, i, j) = F(o->Dvadv, i, j) + F(o->Dvpres, i, j);
         }
-    ROWS
-    for (int j = 1; j <= ny; j++)
-        for (int i = 1; i <= nx; i++) { /* dissipation_effect, calc.c:396-397 */
-            F(Dun, i, j) -= F(o->u, i, j) * o->sigma_dis * o->Dt;
-            F(Dvn, i, j) -= F(o->v, i, j) * o->sigma_dis * o->Dt;
-        }
+    dissipation(o, Dun, Dvn);
     if (o->nu2 != 0 || o->nu4 != 0 || o->nu6 != 0) {
         double *zs[] = {o->Lu, o->Lv, o->LLu, o->LLv, o->LLLu, o->LLLv};
         for (int q = 0; q < 6; q++) memset(zs[q], 0, sizeof(double) * o->S);
@@ -908,7 +964,7 @@ static int step_once(orc *o) {
             laplacian_vels_o(o, o->Lu, o->Lv, o->LLu, o->LLv);
             if (o->nu6 != 0) laplacian_vels_o(o, o->LLu, o->LLv, o->LLLu, o->LLLv);
         }
-        ROWS
+ROWS
         for (int j = 1; j <= ny; j++)
             for (int i = 1; i <= nx; i++) {
                 F(Dun, i, j) += (o->nu2) * (o->Dt) * F(o->Lu, i, j);
@@ -923,23 +979,8 @@ static int step_once(orc *o) {
     adams_bashforth(o, o->Dvab, Dvn, o->Dv[1], o->Dv[2], k);
     rotate(o->Du);
     rotate(o->Dv);
-    /* Coriolis, u then v, in place (calc.c:542-598) */
-    ROWS
-    for (int j = 1; j <= ny; j++)
-        for (int i = 1; i <= nx; i++) {
-            double semiU = 0.25 * (F(o->v, i, j) + F(o->v, i + 1, j) + F(o->v, i, j - 1) + F(o->v, i + 1, j - 1));
-            double Alpha = Dt * o->fe[RJ(j)] + (Dt * F(o->u, i, j) * sin(LATC(j))) / RZ(LATC(j));
-            double Beta = 0.25 * Alpha * Alpha;
-            F(o->u, i, j) = (F(o->u, i, j) + F(o->Duab, i, j) - Beta * F(o->u, i, j) + Alpha * semiU) / (1.0 + Beta);
-        }
-    ROWS
-    for (int j = 1; j <= ny; j++)
-        for (int i = 1; i <= nx; i++) {
-            double semiV = 0.25 * (F(o->u, i - 1, j + 1) + F(o->u, i, j + 1) + F(o->u, i, j) + F(o->u, i - 1, j));
-            double Alpha = Dt * o->fn[RJ(j)] + (Dt * semiV * sin(LATN(j))) / RZ(LATN(j));
-            double Beta = 0.25 * Alpha * Alpha;
-            F(o->v, i, j) = (F(o->v, i, j) + F(o->Dvab, i, j) - Beta * F(o->v, i, j) - Alpha * semiV) / (1.0 + Beta);
-        }
+    coriolis_u(o, o->Duab, Dt);
+    coriolis_v(o, o->Dvab, Dt);
     halo_update_o(o, o->u);
     halo_update_o(o, o->v);
     /* perturbations (core.c:292-300) */
@@ -998,3 +1039,22 @@ void orc_apply_laplacian_eta(orc *o, double *ein, double *eout) {
     halo_update_o(o, ein);
     laplacian_eta_o(o, ein, eout);
 }
+
+/* ---------------- SW2's per-step functions on the oracle's fields --------
+ * (sw.h:229-297): the same code the oracle's step runs, for checking the
+ * library's per-step entry points one by one.  `out` arrays are full haloed
+ * fields; only the cells the reference function writes are written. */
+void orc_solve_tracer(orc *o, double *out, double Dt) { DISPATCH(body_tracer, out, Dt) }
+void orc_solve_cons_h(orc *o, double *out, double Dt) { cons_h(o, out, Dt); }
+void orc_do_eta_geofactor(orc *o, double *d, double Dt) { geofactor(o, d, Dt); }
+void orc_solve_adv_u(orc *o, double *out, double Dt) { DISPATCH(body_u, out, Dt) }
+void orc_solve_adv_v(orc *o, double *out, double Dt) { DISPATCH(body_v, out, Dt) }
+void orc_solve_pres_u(orc *o, double *out, double Dt) { pres_u(o, out, Dt); }
+void orc_solve_pres_v(orc *o, double *out, double Dt) { pres_v(o, out, Dt); }
+void orc_solve_coriolis_u(orc *o, const double *duab, double Dt) { coriolis_u(o, duab, Dt); }
+void orc_solve_coriolis_v(orc *o, const double *dvab, double Dt) { coriolis_v(o, dvab, Dt); }
+void orc_dissipation_effect(orc *o, double *du, double *dv) { dissipation(o, du, dv); }
+void orc_adams_bashforth(orc *o, double *nab, double *d, double *m1, double *m2, int k) { adams_bashforth(o, nab, d, m1, m2, k); }
+double orc_compute_dt(orc *o, double Courant) { return compute_dt_c(o, Courant); }
+void orc_laplacian_vels(orc *o, double *uin, double *vin, double *uout, double *vout) { laplacian_vels_o(o, uin, vin, uout, vout); }
+void orc_laplacian_eta(orc *o, double *ein, double *eout) { laplacian_eta_o(o, ein, eout); }

@@ -257,6 +257,11 @@ class Batch:
     def sync(self):
         _check(lib().sw2_batch_sync(self.h), "sw2_batch_sync")
 
+    def pause(self, mid, paused=True):
+        """Hold a member between steps (its tiles skipped, its step count
+        kept) or release it."""
+        _check(lib().sw2_batch_pause(self.h, mid, 1 if paused else 0), "sw2_batch_pause")
+
     def member(self, mid):
         a, b, c = c_int(), c_int(), c_int()
         _check(lib().sw2_batch_member(self.h, mid, byref(a), byref(b), byref(c)), "sw2_batch_member")

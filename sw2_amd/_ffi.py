@@ -79,12 +79,15 @@ EXPORTS = [
     "cr_reset", "cr_start", "cr_end", "cr_time", "cr_info",
     "sw2_init_params", "sw2_read_case", "sw2_free_params", "sw2_eval_expr",
     "halo_update", "do_channel", "calc_energy_balance", "sw2_mass",
+    "solve_tracer", "solve_cons_h", "dissipation_effect", "do_eta_geofactor", "solve_adv_u", "solve_adv_v",
+    "solve_pres_u", "solve_pres_v", "solve_Coriolis_u", "solve_Coriolis_v", "do_AdamsBashforth", "compute_Dt",
+    "laplacian_vels", "laplacian_eta",
     "save_binary_sw", "load_binary_sw",
     "sw2_session_create", "sw2_session_begin", "sw2_session_step", "sw2_session_time", "sw2_session_kernel_times",
     "sw2_session_energy", "sw2_session_download", "sw2_session_form",
     "sw2_session_failed_step", "sw2_session_last_cfl", "sw2_session_fused", "sw2_session_state", "sw2_session_destroy",
     "sw2_device_cus", "sw2_batch_footprint", "sw2_batch_create", "sw2_batch_add", "sw2_batch_set_launch", "sw2_batch_get_launch",
-    "sw2_batch_step", "sw2_batch_sync", "sw2_batch_member", "sw2_batch_finished", "sw2_batch_download",
+    "sw2_batch_step", "sw2_batch_sync", "sw2_batch_pause", "sw2_batch_member", "sw2_batch_finished", "sw2_batch_download",
     "sw2_batch_energy", "sw2_batch_remove", "sw2_batch_count", "sw2_batch_destroy",
     "sw2_last_error", "sw2_build_info",
 ]
@@ -111,6 +114,20 @@ def lib():
     L.calc_energy_balance.argtypes = [P, P, POINTER(SW)]
     L.sw2_mass.argtypes = [POINTER(SW)]
     L.sw2_mass.restype = c_double
+    for name in ("solve_tracer", "solve_cons_h", "do_eta_geofactor", "solve_adv_u", "solve_adv_v", "solve_pres_u",
+                 "solve_pres_v", "solve_Coriolis_u", "solve_Coriolis_v"):
+        getattr(L, name).argtypes = [P, c_double, POINTER(SW)]
+        getattr(L, name).restype = None
+    L.dissipation_effect.argtypes = [P, P, POINTER(SW)]
+    L.dissipation_effect.restype = None
+    L.do_AdamsBashforth.argtypes = [P, P, P, P, c_int, POINTER(Map)]
+    L.do_AdamsBashforth.restype = None
+    L.compute_Dt.argtypes = [c_double, POINTER(SW)]
+    L.compute_Dt.restype = c_double
+    L.laplacian_vels.argtypes = [P, P, P, P, POINTER(SW)]
+    L.laplacian_vels.restype = None
+    L.laplacian_eta.argtypes = [P, P, POINTER(SW)]
+    L.laplacian_eta.restype = None
     L.destroy.argtypes = [POINTER(SW)]
     L.save_binary_sw.argtypes = [c_int, c_int, c_double, ctypes.c_char_p, ctypes.c_char_p, POINTER(SW)]
     L.load_binary_sw.argtypes = [c_int, POINTER(c_int), POINTER(c_double), ctypes.c_char_p, ctypes.c_char_p, POINTER(SW)]
@@ -138,6 +155,7 @@ def lib():
     L.sw2_batch_get_launch.argtypes = [c_void_p, PI, PI, PI]
     L.sw2_batch_step.argtypes = [c_void_p, c_int]
     L.sw2_batch_sync.argtypes = [c_void_p]
+    L.sw2_batch_pause.argtypes = [c_void_p, c_int, c_int]
     L.sw2_batch_member.argtypes = [c_void_p, c_int, PI, PI, PI]
     L.sw2_batch_finished.argtypes = [c_void_p, PI, c_int]
     L.sw2_batch_download.argtypes = [c_void_p, c_int, POINTER(SW)]

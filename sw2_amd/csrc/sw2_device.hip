@@ -15,6 +15,7 @@
 // correctly rounded; transcendental row values come from the host libm.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stdarg.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -257,6 +258,69 @@ __global__ __launch_bounds__(BX *BY) void k_tracer_update(Grid g, Flags *fl, dou
 }
 
 // ------------------------------------------------ momentum tendencies (BODYU, BODYV + pressure + drag + hypervisc + AB)
+// advection of u at (i, j): BODYU, calc.c:168-200 (solve_adv_u, calc.c:424-460)
+template <int SCH>
+__device__ __forceinline__ double adv_u(const Grid &g, const Ptrs &p, int i, int j, double Dt) {
+    double dxc = TB(g, SW2_T_DXC, j), dyv = TB(g, SW2_T_DYV, j), dxe = TB(g, SW2_T_DXE, j);
+    double a0 = UW(i - 2, j), a1 = UW(i - 1, j), a2 = UW(i, j), a3 = UW(i + 1, j), a4 = UW(i + 2, j);
+    double c0 = UW(i, j - 2), c1 = UW(i, j - 1), c3 = UW(i, j + 1), c4_ = UW(i, j + 2);
+    double vs0 = VW(i, j - 1), vs1 = VW(i + 1, j - 1), vn0 = VW(i, j), vn1 = VW(i + 1, j);
+    double CwP = ddiv(0.5 * (0.5 * (a1 + fabs(a1)) + 0.5 * (a2 + fabs(a2))) * Dt, dxc);
+    double CwM = ddiv(0.5 * (0.5 * (a1 - fabs(a1)) + 0.5 * (a2 - fabs(a2))) * Dt, dxc);
+    double CeP = ddiv(0.5 * (0.5 * (a2 + fabs(a2)) + 0.5 * (a3 + fabs(a3))) * Dt, dxc);
+    double CeM = ddiv(0.5 * (0.5 * (a2 - fabs(a2)) + 0.5 * (a3 - fabs(a3))) * Dt, dxc);
+    double CsP = ddiv(0.5 * (0.5 * (vs0 + fabs(vs0)) + 0.5 * (vs1 + fabs(vs1))) * Dt, dyv);
+    double CsM = ddiv(0.5 * (0.5 * (vs0 - fabs(vs0)) + 0.5 * (vs1 - fabs(vs1))) * Dt, dyv);
+    double CnP = ddiv(0.5 * (0.5 * (vn0 + fabs(vn0)) + 0.5 * (vn1 + fabs(vn1))) * Dt, dyv);
+    double CnM = ddiv(0.5 * (0.5 * (vn0 - fabs(vn0)) + 0.5 * (vn1 - fabs(vn1))) * Dt, dyv);
+    double rxPprev = r2rP(a0, a1, a2), rxMprev = r2rM(a1, a2, a3), rxP = r2rP(a1, a2, a3), rxM = r2rM(a2, a3, a4);
+    double ryPprev = r2rP(c0, c1, a2), ryMprev = r2rM(c1, a2, c3), ryP = r2rP(c1, a2, c3), ryM = r2rM(a2, c3, c4_);
+    double BwP = b2bp<SCH>(a1, a2, rxPprev, CwP), BwM = b2bm<SCH>(a1, a2, rxMprev, CwM);
+    double BeP = b2bp<SCH>(a2, a3, rxP, CeP), BeM = b2bm<SCH>(a2, a3, rxM, CeM);
+    double BsP = b2bp<SCH>(c1, a2, ryPprev, CsP), BsM = b2bm<SCH>(c1, a2, ryMprev, CsM);
+    double BnP = b2bp<SCH>(a2, c3, ryP, CnP), BnM = b2bm<SCH>(a2, c3, ryM, CnM);
+    double p1 = CwP * BwP + CwM * BwM - CeP * BeP - CeM * BeM + CsP * BsP + CsM * BsM - CnP * BnP - CnM * BnM;
+    double p2 = Dt * (ddiv(a2 * (a3 - a1), (dxe + dxe)) + ddiv(a2 * ((vn0 + vn1) / 2. - (vs0 + vs1) / 2.), dyv));
+    return p1 + p2;
+}
+// pressure gradient of u (solve_pres_u, calc.c:500-519)
+__device__ __forceinline__ double pres_u(const Grid &g, const Ptrs &p, int i, int j, double Dt) {
+    const int q = IX(g, i, j);
+    const double dxc = TB(g, SW2_T_DXC, j);
+    return ddiv(-TB(g, SW2_T_GE, j) * Dt * (p.eta[IX(g, i + 1, j)] - p.eta[q]), dxc);  // calc.c:517
+}
+// advection of v at (i, j): BODYV, calc.c:205-237 (solve_adv_v, calc.c:462-498)
+template <int SCH>
+__device__ __forceinline__ double adv_v(const Grid &g, const Ptrs &p, int i, int j, double Dt) {
+    double dxv = TB(g, SW2_T_DXV, j), dyc = TB(g, SW2_T_DYC, j);
+    double dyn2 = TB(g, SW2_T_DYN, j) + TB(g, SW2_T_DYN, j + 1);
+    double uw_m1j1 = UW(i - 1, j + 1), uw_m1j = UW(i - 1, j), uw_j1 = UW(i, j + 1), uw_j = UW(i, j);
+    double a0 = VW(i - 2, j), a1 = VW(i - 1, j), a2 = VW(i, j), a3 = VW(i + 1, j), a4 = VW(i + 2, j);
+    double c0 = VW(i, j - 2), c1 = VW(i, j - 1), c3 = VW(i, j + 1), c4_ = VW(i, j + 2);
+    double CwP = ddiv(0.5 * (0.5 * (uw_m1j1 + fabs(uw_m1j1)) + 0.5 * (uw_m1j + fabs(uw_m1j))) * Dt, dxv);
+    double CwM = ddiv(0.5 * (0.5 * (uw_m1j1 - fabs(uw_m1j1)) + 0.5 * (uw_m1j - fabs(uw_m1j))) * Dt, dxv);
+    double CeP = ddiv(0.5 * (0.5 * (uw_j1 + fabs(uw_j1)) + 0.5 * (uw_j + fabs(uw_j))) * Dt, dxv);
+    double CeM = ddiv(0.5 * (0.5 * (uw_j1 - fabs(uw_j1)) + 0.5 * (uw_j - fabs(uw_j))) * Dt, dxv);
+    double CsP = ddiv(0.5 * (0.5 * (c1 + fabs(c1)) + 0.5 * (a2 + fabs(a2))) * Dt, dyc);
+    double CsM = ddiv(0.5 * (0.5 * (c1 - fabs(c1)) + 0.5 * (a2 - fabs(a2))) * Dt, dyc);
+    double CnP = ddiv(0.5 * (0.5 * (a2 + fabs(a2)) + 0.5 * (c3 + fabs(c3))) * Dt, dyc);
+    double CnM = ddiv(0.5 * (0.5 * (a2 - fabs(a2)) + 0.5 * (c3 - fabs(c3))) * Dt, dyc);
+    double rxPprev = r2rP(a0, a1, a2), rxMprev = r2rM(a1, a2, a3), rxP = r2rP(a1, a2, a3), rxM = r2rM(a2, a3, a4);
+    double ryPprev = r2rP(c0, c1, a2), ryMprev = r2rM(c1, a2, c3), ryP = r2rP(c1, a2, c3), ryM = r2rM(a2, c3, c4_);
+    double BwP = b2bp<SCH>(a1, a2, rxPprev, CwP), BwM = b2bm<SCH>(a1, a2, rxMprev, CwM);
+    double BeP = b2bp<SCH>(a2, a3, rxP, CeP), BeM = b2bm<SCH>(a2, a3, rxM, CeM);
+    double BsP = b2bp<SCH>(c1, a2, ryPprev, CsP), BsM = b2bm<SCH>(c1, a2, ryMprev, CsM);
+    double BnP = b2bp<SCH>(a2, c3, ryP, CnP), BnM = b2bm<SCH>(a2, c3, ryM, CnM);
+    double p1 = CwP * BwP + CwM * BwM - CeP * BeP - CeM * BeM + CsP * BsP + CsM * BsM - CnP * BnP - CnM * BnM;
+    double p2 = Dt * (ddiv(a2 * ((uw_j + uw_j1) / 2. - (uw_m1j + uw_m1j1) / 2.), dxv) + ddiv(a2 * (c3 - c1), dyn2));
+    return p1 + p2;
+}
+// pressure gradient of v (solve_pres_v, calc.c:521-540)
+__device__ __forceinline__ double pres_v(const Grid &g, const Ptrs &p, int i, int j, double Dt) {
+    const int q = IX(g, i, j);
+    return ddiv(-TB(g, SW2_T_GN, j) * Dt * (p.eta[IX(g, i, j + 1)] - p.eta[q]), TB(g, SW2_T_DYC, j + 1));  // calc.c:538
+}
+
 template <int SCH>
 __global__ __launch_bounds__(BX *BY) void k_uv(Grid g, Ptrs p, Flags *fl, double Dt, double sigma, int hv, double c2,
                                                  double c4, double c6, const double *Lu, const double *Lv,
@@ -267,55 +331,14 @@ __global__ __launch_bounds__(BX *BY) void k_uv(Grid g, Ptrs p, Flags *fl, double
     CELL();
     const int q = IX(g, i, j);
     double du, dv;
-    {  // BODYU, calc.c:168-200
-        double dxc = TB(g, SW2_T_DXC, j), dyv = TB(g, SW2_T_DYV, j), dxe = TB(g, SW2_T_DXE, j);
-        double a0 = UW(i - 2, j), a1 = UW(i - 1, j), a2 = UW(i, j), a3 = UW(i + 1, j), a4 = UW(i + 2, j);
-        double c0 = UW(i, j - 2), c1 = UW(i, j - 1), c3 = UW(i, j + 1), c4_ = UW(i, j + 2);
-        double vs0 = VW(i, j - 1), vs1 = VW(i + 1, j - 1), vn0 = VW(i, j), vn1 = VW(i + 1, j);
-        double CwP = ddiv(0.5 * (0.5 * (a1 + fabs(a1)) + 0.5 * (a2 + fabs(a2))) * Dt, dxc);
-        double CwM = ddiv(0.5 * (0.5 * (a1 - fabs(a1)) + 0.5 * (a2 - fabs(a2))) * Dt, dxc);
-        double CeP = ddiv(0.5 * (0.5 * (a2 + fabs(a2)) + 0.5 * (a3 + fabs(a3))) * Dt, dxc);
-        double CeM = ddiv(0.5 * (0.5 * (a2 - fabs(a2)) + 0.5 * (a3 - fabs(a3))) * Dt, dxc);
-        double CsP = ddiv(0.5 * (0.5 * (vs0 + fabs(vs0)) + 0.5 * (vs1 + fabs(vs1))) * Dt, dyv);
-        double CsM = ddiv(0.5 * (0.5 * (vs0 - fabs(vs0)) + 0.5 * (vs1 - fabs(vs1))) * Dt, dyv);
-        double CnP = ddiv(0.5 * (0.5 * (vn0 + fabs(vn0)) + 0.5 * (vn1 + fabs(vn1))) * Dt, dyv);
-        double CnM = ddiv(0.5 * (0.5 * (vn0 - fabs(vn0)) + 0.5 * (vn1 - fabs(vn1))) * Dt, dyv);
-        double rxPprev = r2rP(a0, a1, a2), rxMprev = r2rM(a1, a2, a3), rxP = r2rP(a1, a2, a3), rxM = r2rM(a2, a3, a4);
-        double ryPprev = r2rP(c0, c1, a2), ryMprev = r2rM(c1, a2, c3), ryP = r2rP(c1, a2, c3), ryM = r2rM(a2, c3, c4_);
-        double BwP = b2bp<SCH>(a1, a2, rxPprev, CwP), BwM = b2bm<SCH>(a1, a2, rxMprev, CwM);
-        double BeP = b2bp<SCH>(a2, a3, rxP, CeP), BeM = b2bm<SCH>(a2, a3, rxM, CeM);
-        double BsP = b2bp<SCH>(c1, a2, ryPprev, CsP), BsM = b2bm<SCH>(c1, a2, ryMprev, CsM);
-        double BnP = b2bp<SCH>(a2, c3, ryP, CnP), BnM = b2bm<SCH>(a2, c3, ryM, CnM);
-        double p1 = CwP * BwP + CwM * BwM - CeP * BeP - CeM * BeM + CsP * BsP + CsM * BsM - CnP * BnP - CnM * BnM;
-        double p2 = Dt * (ddiv(a2 * (a3 - a1), (dxe + dxe)) + ddiv(a2 * ((vn0 + vn1) / 2. - (vs0 + vs1) / 2.), dyv));
-        double adv = p1 + p2;
-        double pres = ddiv(-TB(g, SW2_T_GE, j) * Dt * (p.eta[IX(g, i + 1, j)] - p.eta[q]), dxc);  // calc.c:517
-        du = adv + pres;                                                                      // core.c:224
+    {
+        double adv = adv_u<SCH>(g, p, i, j, Dt);
+        double pres = pres_u(g, p, i, j, Dt);
+        du = adv + pres;  // core.c:224
     }
-    {  // BODYV, calc.c:205-237
-        double dxv = TB(g, SW2_T_DXV, j), dyc = TB(g, SW2_T_DYC, j);
-        double dyn2 = TB(g, SW2_T_DYN, j) + TB(g, SW2_T_DYN, j + 1);
-        double uw_m1j1 = UW(i - 1, j + 1), uw_m1j = UW(i - 1, j), uw_j1 = UW(i, j + 1), uw_j = UW(i, j);
-        double a0 = VW(i - 2, j), a1 = VW(i - 1, j), a2 = VW(i, j), a3 = VW(i + 1, j), a4 = VW(i + 2, j);
-        double c0 = VW(i, j - 2), c1 = VW(i, j - 1), c3 = VW(i, j + 1), c4_ = VW(i, j + 2);
-        double CwP = ddiv(0.5 * (0.5 * (uw_m1j1 + fabs(uw_m1j1)) + 0.5 * (uw_m1j + fabs(uw_m1j))) * Dt, dxv);
-        double CwM = ddiv(0.5 * (0.5 * (uw_m1j1 - fabs(uw_m1j1)) + 0.5 * (uw_m1j - fabs(uw_m1j))) * Dt, dxv);
-        double CeP = ddiv(0.5 * (0.5 * (uw_j1 + fabs(uw_j1)) + 0.5 * (uw_j + fabs(uw_j))) * Dt, dxv);
-        double CeM = ddiv(0.5 * (0.5 * (uw_j1 - fabs(uw_j1)) + 0.5 * (uw_j - fabs(uw_j))) * Dt, dxv);
-        double CsP = ddiv(0.5 * (0.5 * (c1 + fabs(c1)) + 0.5 * (a2 + fabs(a2))) * Dt, dyc);
-        double CsM = ddiv(0.5 * (0.5 * (c1 - fabs(c1)) + 0.5 * (a2 - fabs(a2))) * Dt, dyc);
-        double CnP = ddiv(0.5 * (0.5 * (a2 + fabs(a2)) + 0.5 * (c3 + fabs(c3))) * Dt, dyc);
-        double CnM = ddiv(0.5 * (0.5 * (a2 - fabs(a2)) + 0.5 * (c3 - fabs(c3))) * Dt, dyc);
-        double rxPprev = r2rP(a0, a1, a2), rxMprev = r2rM(a1, a2, a3), rxP = r2rP(a1, a2, a3), rxM = r2rM(a2, a3, a4);
-        double ryPprev = r2rP(c0, c1, a2), ryMprev = r2rM(c1, a2, c3), ryP = r2rP(c1, a2, c3), ryM = r2rM(a2, c3, c4_);
-        double BwP = b2bp<SCH>(a1, a2, rxPprev, CwP), BwM = b2bm<SCH>(a1, a2, rxMprev, CwM);
-        double BeP = b2bp<SCH>(a2, a3, rxP, CeP), BeM = b2bm<SCH>(a2, a3, rxM, CeM);
-        double BsP = b2bp<SCH>(c1, a2, ryPprev, CsP), BsM = b2bm<SCH>(c1, a2, ryMprev, CsM);
-        double BnP = b2bp<SCH>(a2, c3, ryP, CnP), BnM = b2bm<SCH>(a2, c3, ryM, CnM);
-        double p1 = CwP * BwP + CwM * BwM - CeP * BeP - CeM * BeM + CsP * BsP + CsM * BsM - CnP * BnP - CnM * BnM;
-        double p2 = Dt * (ddiv(a2 * ((uw_j + uw_j1) / 2. - (uw_m1j + uw_m1j1) / 2.), dxv) + ddiv(a2 * (c3 - c1), dyn2));
-        double adv = p1 + p2;
-        double pres = ddiv(-TB(g, SW2_T_GN, j) * Dt * (p.eta[IX(g, i, j + 1)] - p.eta[q]), TB(g, SW2_T_DYC, j + 1));  // calc.c:538
+    {
+        double adv = adv_v<SCH>(g, p, i, j, Dt);
+        double pres = pres_v(g, p, i, j, Dt);
         dv = adv + pres;
     }
     du -= p.u[q] * sigma * Dt;  // calc.c:396-397
@@ -1448,7 +1471,7 @@ extern "C" int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id) {
     b->nmem++;
     batch_retile(b);
     // a fresh guard record and the CFL maxima of the member's first step
-    BState z{0, 0};
+    BState z{0, 0, 0};
     HIPCHECK(hipMemsetAsync(s->fl, 0, sizeof(Flags), b->st));
     km_cfl<<<256, 256, 0, b->st>>>(s->g, s->fl, s->m_u, s->m_v, 1);
     HIPCHECK(hipMemcpyAsync(b->d_st + slot, &z, sizeof z, hipMemcpyHostToDevice, b->st));
@@ -1560,6 +1583,20 @@ extern "C" int sw2_batch_member(sw2_batch *b, int id, int *steps_done, int *done
     return 0;
 }
 
+// hold a member (its tiles skipped, its step count kept) or release it;
+// between steps only: the caller has synchronised the batch
+extern "C" int sw2_batch_pause(sw2_batch *b, int id, int paused) {
+    if (id < 0 || id >= b->cap || !b->sess[id]) {
+        sw2_set_error("no member %d", id);
+        return -1;
+    }
+    const int v = paused ? 1 : 0;
+    HIPCHECK(hipMemcpyAsync(reinterpret_cast<char *>(b->d_st + id) + offsetof(BState, paused), &v, sizeof v,
+                            hipMemcpyHostToDevice, b->st));
+    HIPCHECK(hipStreamSynchronize(b->st));
+    return 0;
+}
+
 // ids of the members whose runs ended (last step reached or a guard failed)
 extern "C" int sw2_batch_finished(sw2_batch *b, int *ids, int maxids) {
     BState *h = (BState *)malloc(sizeof(BState) * b->cap);
@@ -1611,7 +1648,7 @@ extern "C" int sw2_batch_remove(sw2_batch *b, int id) {
         return -1;
     }
     HIPCHECK(hipStreamSynchronize(b->st));
-    BState z{0, 1};
+    BState z{0, 1, 0};
     HIPCHECK(hipMemcpy(b->d_st + id, &z, sizeof z, hipMemcpyHostToDevice));
     memset(&b->h_mem[id], 0, sizeof(BMember));
     batch_retile(b);
@@ -1647,3 +1684,5 @@ extern "C" void sw2_batch_destroy(sw2_batch *b) {
     free(b->h_mem);
     free(b);
 }
+
+#include "sw2_fine.inc"

@@ -250,6 +250,13 @@ static void coords(sw *SW) {
     }
 }
 
+void sw2_geofactor_rows(sw *SW, double Dt, double *gf) {
+    for (int j = -1; j <= SW->ny + 2; j++) {
+        double latc = g_latc(SW, j);
+        gf[j + 1] = Dt * sin(latc) / g_rZ(SW, latc);
+    }
+}
+
 int sw2_row_tables(sw *SW, double *tab) {
     int R = SW->ny + 4;
     map *M = SW->M;

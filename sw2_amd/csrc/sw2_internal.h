@@ -58,6 +58,10 @@ int sw2_gaussians_at(sw *SW, double Dt, double t, int maxk, double **inc, double
  * eta/u/v order into deta[k], du[k], dv[k]. */
 int sw2_vortices_at(sw *SW, double Dt, double t, int maxk, double **deta, double **du, double **dv);
 
+/* gf(j) = Dt*sin(latc)/rZ(latc) of do_eta_geofactor (calc.c:420) for the
+ * rows j = -1..ny+2 (index j+1), computed with the host libm as SW2 does. */
+void sw2_geofactor_rows(sw *SW, double Dt, double *gf);
+
 /* Check whether a field is identically zero (halos included). */
 int sw2_all_zero(const double *f, int n);
 

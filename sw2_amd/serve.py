@@ -12,6 +12,14 @@ world (geometry, wind table, storm bumps) is built on the host by a small
 pool of builder threads as soon as it arrives, while the device runs the
 current tick, so admission only uploads it.
 
+Scheduling.  With policy "srpt" (shortest remaining processing time, the
+default) every tick advances the admitted runs with the least remaining
+work (steps left x cells), as many as it takes to give the persistent
+launches `fill` tiles per resident workgroup, and holds the rest
+(sw2_batch_pause): under overload short runs finish early instead of all
+runs finishing late.  Policy "all" advances every admitted run every tick
+(processor sharing).
+
 Reported: time-to-completion (arrival to retirement) p50 / p99, runs per
 hour, cell-updates per second and the fraction of time the device had work.
 """
@@ -31,13 +39,18 @@ def _build(req):
 
 
 class Server:
-    def __init__(self, capacity=64, hbm_budget=0.0, tick_steps=40, launch=None, builders=2, prefetch=8):
+    def __init__(self, capacity=64, hbm_budget=0.0, tick_steps=40, launch=None, builders=2, prefetch=8,
+                 policy="srpt", fill=4):
+        if policy not in ("srpt", "all"):
+            raise ValueError(f"policy {policy!r}: 'srpt' or 'all'")
         self.batch = Batch(capacity=capacity, hbm_budget=hbm_budget)
         if launch:
             self.batch.set_launch(*launch)
         self.tick = tick_steps
         self.pool = ThreadPoolExecutor(max_workers=builders)
         self.prefetch = prefetch
+        self.policy = policy
+        self.fill_tiles = fill * max(self.batch.launch()[:2])
 
     def close(self):
         self.pool.shutdown(wait=True)
@@ -58,6 +71,8 @@ class Server:
         reqs = sorted(enumerate(requests), key=lambda r: r[1][0])
         waiting = deque()  # [request index, request, future or None]
         live = {}  # member id -> (request index, arrival, admitted)
+        left = {}  # member id -> (steps left, cells, tiles)
+        held = set()
         done = {}
         rejected = []
         cells_steps = 0.0
@@ -91,12 +106,14 @@ class Server:
                     break
                 waiting.popleft()
                 live[mid] = (k, ta, time.perf_counter() - t0)
+                left[mid] = (steps, nx * ny, ((nx + 63) // 64 + 1) * ((ny + 7) // 8))
                 cells_steps += float(nx) * ny * steps
             self._prefetch(waiting)
             if not live:
                 if nxt < len(reqs):
                     time.sleep(max(0.0, min(0.05, reqs[nxt][1][0] - (time.perf_counter() - t0))))
                 continue
+            active = self._schedule(left, held)
             tb = time.perf_counter()
             self.batch.step(self.tick)
             fin = self.batch.finished()  # waits for the tick
@@ -104,8 +121,13 @@ class Server:
             tnow = time.perf_counter() - t0
             if log and fin:
                 log(f"t={tnow:.1f}s: {len(fin)} done, {len(done) + len(fin)}/{len(requests)} completed, "
-                    f"{len(live) - len(fin)} running, {len(waiting)} waiting")
+                    f"{len(live) - len(fin)} admitted ({len(active)} advancing), {len(waiting)} waiting")
+            for mid in active:
+                st_, c_, t_ = left[mid]
+                left[mid] = (max(0, st_ - self.tick), c_, t_)
             for mid in fin:
+                left.pop(mid, None)
+                held.discard(mid)
                 k, ta, tadm = live.pop(mid)
                 st = self.batch.member(mid)
                 if on_result:
@@ -127,4 +149,28 @@ class Server:
             "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "queue_wait_p50_s": float(np.percentile(wait, 50)),
             "cell_updates_per_s": cells_steps / wall, "device_busy_frac": busy / wall,
+            "policy": self.policy,
         }
+
+    def _schedule(self, left, held):
+        """Choose the runs the next tick advances and hold the others;
+        returns the advancing member ids."""
+        ids = sorted(left, key=lambda m: (left[m][0] * left[m][1], m))
+        if self.policy == "all":
+            active = ids
+        else:
+            active, tiles = [], 0
+            for m in ids:
+                if tiles >= self.fill_tiles:
+                    break
+                active.append(m)
+                tiles += left[m][2]
+        on = set(active)
+        for m in ids:
+            if m in on and m in held:
+                self.batch.pause(m, False)
+                held.discard(m)
+            elif m not in on and m not in held:
+                self.batch.pause(m, True)
+                held.add(m)
+        return active
