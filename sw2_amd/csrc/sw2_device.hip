@@ -594,6 +594,7 @@ struct sw2_session {
     float *m_eta, *m_u, *m_v, *m_eta2, *m_u2, *m_v2, *m_De[3], *m_Du[3], *m_Dv[3];
     double *d_mt, *h_hbar;
     double *d_part;          // energy reduction scratch
+    unsigned long long *d_cflp;  // partial CFL maxima when stepped in a batch
     hipEvent_t *kev;         // per-kernel timing (sw2_session_kernel_times): 3 events a step
     int kev_i;
     double t, Dt, depth, sigma, tracer_dis, courant, rest_x, rest_y;
@@ -1022,7 +1023,7 @@ static void launch_mixed(sw2_session *s, int n, int k, int slot) {
     km_eta<SCH, HV><<<G, NTE, 0, s->st>>>(s->g, s->fl, ea);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 1], s->st);
     MUVArgs ua{s->m_eta2, s->m_u, s->m_v, s->m_Du[1], s->m_Du[2], s->m_Dv[1], s->m_Dv[2], s->m_u2, s->m_v2, s->m_Du[0],
-               s->m_Dv[0], s->d_mt, n, k, slot ^ 1, s->Dt, s->sigma, s->c2, s->c4};
+               s->m_Dv[0], s->d_mt, &s->fl->cfl[2 * (slot ^ 1)], n, k, s->Dt, s->sigma, s->c2, s->c4};
     km_uv<SCH, HV><<<G, NTU, 0, s->st>>>(s->g, s->fl, ua);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 2], s->st);
 }
@@ -1292,6 +1293,7 @@ extern "C" void sw2_session_destroy(sw2_session *s) {
     }
     if (s->d_mt) (void)hipFree(s->d_mt);
     if (s->d_part) (void)hipFree(s->d_part);
+    if (s->d_cflp) (void)hipFree(s->d_cflp);
     free(s->h_hbar);
     if (s->fl) (void)hipFree(s->fl);
     if (s->st) (void)hipStreamDestroy(s->st);
@@ -1441,6 +1443,11 @@ extern "C" int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id) {
     memset(&M, 0, sizeof M);
     M.g = s->g;
     M.fl = s->fl;
+    if (!s->d_cflp) {
+        HIPCHECK(hipMalloc((void **)&s->d_cflp, sizeof(unsigned long long) * 2 * NPART));
+        HIPCHECK(hipMemset(s->d_cflp, 0, sizeof(unsigned long long) * 2 * NPART));
+    }
+    M.cflp = s->d_cflp;
     M.mt = s->d_mt;
     M.eta[0] = s->m_eta;
     M.eta[1] = s->m_eta2;
@@ -1505,7 +1512,7 @@ static void batch_launch_step(sw2_batch *b) {
             BATCH_LAUNCH(9)
     }
 #undef BATCH_LAUNCH
-    kb_adv<<<1, 256, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);
+    kb_adv<<<(b->cap + 3) / 4, 256, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);
 }
 
 static int batch_upload(sw2_batch *b) {
