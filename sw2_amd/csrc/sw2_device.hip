@@ -660,6 +660,8 @@ static int mixed_setup(sw2_session *s, sw *SW) {
     free(mt);
     float **fs[] = {&s->m_eta, &s->m_u, &s->m_v, &s->m_eta2, &s->m_u2, &s->m_v2};
     for (auto f : fs) rc |= falloc(f, S);
+    if (!rc && hipMalloc((void **)&s->d_cflp, sizeof(unsigned long long) * 2 * NPART) != hipSuccess) rc = -1;
+    if (!rc && hipMemset(s->d_cflp, 0, sizeof(unsigned long long) * 2 * NPART) != hipSuccess) rc = -1;
     for (int k = 0; k < 3; k++) {
         rc |= falloc(&s->m_De[k], S);
         rc |= falloc(&s->m_Du[k], S);
@@ -1023,9 +1025,10 @@ static void launch_mixed(sw2_session *s, int n, int k, int slot) {
     km_eta<SCH, HV><<<G, NTE, 0, s->st>>>(s->g, s->fl, ea);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 1], s->st);
     MUVArgs ua{s->m_eta2, s->m_u, s->m_v, s->m_Du[1], s->m_Du[2], s->m_Dv[1], s->m_Dv[2], s->m_u2, s->m_v2, s->m_Du[0],
-               s->m_Dv[0], s->d_mt, &s->fl->cfl[2 * (slot ^ 1)], n, k, s->Dt, s->sigma, s->c2, s->c4};
+               s->m_Dv[0], s->d_mt, s->d_cflp, n, k, s->Dt, s->sigma, s->c2, s->c4};
     km_uv<SCH, HV><<<G, NTU, 0, s->st>>>(s->g, s->fl, ua);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 2], s->st);
+    km_cfl_parts<<<1, 64, 0, s->st>>>(s->d_cflp, s->fl, slot ^ 1);
 }
 
 template <int SCH>
