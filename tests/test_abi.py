@@ -39,6 +39,35 @@ def test_library_is_built_for_gfx950():
     assert b"gfx950" in sw2_amd.lib().sw2_build_info()
 
 
+def test_sin_and_cos_stay_separate_libm_calls():
+    """gcc merges sin(x) and cos(x) of one argument into sincos(x) when it
+    sees both (after inlining), and glibc's sincos differs from sin in the
+    last bit for some x: that moved sin(latc) of three rows by an ulp and
+    the 1000-step run of the CPU-runnable case off the oracle.  Neither the
+    library nor the oracle may import sincos, and the table of sin(latc)
+    rows equals libm's sin (Python's math.sin calls it) bit for bit."""
+    import ctypes
+    import math
+    from oracle import LIB_PATH as ORACLE_LIB
+    from sw2_amd import _ffi
+    for path in (LIB_PATH, ORACLE_LIB):
+        out = subprocess.run(["nm", "-D", "--undefined-only", path], capture_output=True, text=True, check=True).stdout
+        assert not any(line.split()[-1].startswith("sincos") for line in out.splitlines() if line.split()), path
+    w, _ = cases.make_world(cases.config_params("c0"), [])
+    L = sw2_amd.lib()
+    nt, R = 64, w.sw.ny + 4
+    tab = np.zeros(nt * R)
+    fn = L.sw2_row_tables
+    fn.argtypes = [ctypes.POINTER(_ffi.SW), ctypes.POINTER(ctypes.c_double)]
+    assert fn(ctypes.byref(w.sw), tab.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 1
+    SINC = 11  # SW2_T_SINC in sw2_internal.h
+    lat0, lat1, ny = w.sw.lat0, w.sw.lat1, w.sw.ny
+    latn = lambda j: lat0 + j * (lat1 - lat0) / ny
+    for j in range(-1, ny + 3):
+        latc = (latn(j) + latn(j - 1)) / 2.0
+        assert tab[SINC * R + j + 1] == math.sin(latc), j
+
+
 @pytest.mark.parametrize("expr,val", [
     ("2+3^2", 11.0), ("-2^2", 4.0), ("2^3^2", 64.0), ("log(100)", 2.0), ("ln(e)", 1.0),
     ("10 % 4", 2.0), ("pow(2,10)", 1024.0), ("sin 0", 0.0), ("(1,2,3)", 3.0), ("86400*20", 1728000.0),
