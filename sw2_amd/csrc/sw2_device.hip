@@ -514,13 +514,17 @@ __global__ void k_add(Grid g, Flags *fl, double *f, const double *inc) {
     f[q] += inc[q];
 }
 
-__global__ void k_assign_masked(int S, Flags *fl, double *tr, double *pert, const double *val, const unsigned char *mask) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (fl->fail_step || q >= S) return;
-    if (mask[q]) {
-        tr[q] = val[q];
-        pert[q] = 1.0;
-    }
+// one storm injection from its sparse list (the cells inside the radius,
+// perturbs.c:348-395): the tracer value and the perturbation mark there, and
+// the eta increment when the injection changes eta
+__global__ void k_inject(Flags *fl, int n, const int *idx, const double *inc, const double *trv, int has_inc, double *eta,
+                         double *tr, double *pert) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (fl->fail_step || k >= n) return;
+    const int q = idx[k];
+    tr[q] = trv[k];
+    pert[q] = 1.0;
+    if (has_inc) eta[q] += inc[k];
 }
 
 __global__ void k_zero(int S, double *f) {
@@ -604,13 +608,15 @@ struct sw2_session {
     double *Deta[3], *Du[3], *Dv[3], *Dtr[3];
     double *duab, *dvab;
     double *Lu, *Lv, *LLu, *LLv, *LLLu, *LLLv, *ze, *di, *le, *lle, *llle, *psi, *xi;
-    double *pinc, *ptr, *pdu, *pdv;
-    unsigned char *pmask;
+    double *pinc, *pdu, *pdv;
     Flags *fl;
     hipStream_t st;
-    // host scratch for perturbations
-    double *h_inc[8], *h_trv[8], *h_dv[3][8];
-    unsigned char *h_mask[8];
+    // host scratch for perturbations: storm injections as sparse lists
+    // (inj, uploaded to d_idx/d_inc/d_trv), vortices as whole fields
+    sw2_inj inj[8];
+    int *d_idx[8], d_cap[8];
+    double *d_inc[8], *d_trv[8];
+    double *h_dv[3][8];
 };
 
 static int dalloc(double **p, size_t n) {
@@ -912,46 +918,64 @@ static int host_perturbations(sw2_session *s, int n, double t, int *eta_changed)
     *eta_changed = 0;
     if (SW->num_Gaussians == 0 && SW->num_vortices == 0) return 0;
     size_t S = (size_t)s->S;
-    if (!s->h_inc[0]) {
-        for (int k = 0; k < 8; k++) {
-            s->h_inc[k] = (double *)malloc(sizeof(double) * S);
-            s->h_trv[k] = (double *)malloc(sizeof(double) * S);
-            s->h_mask[k] = (unsigned char *)malloc(S);
-            for (int q = 0; q < 3; q++) s->h_dv[q][k] = (double *)malloc(sizeof(double) * S);
+    int K = 0;
+    if (SW->num_Gaussians) {
+        // the host evaluates each active injection on the cells inside its
+        // radius (SW2's expressions and libm) while the device runs this
+        // step's kernels; only those cells travel
+        K = sw2_gaussians_sparse(SW, s->Dt, t, 8, s->inj);
+        if (K < 0) return -1;
+        for (int k = 0; k < K; k++) {
+            const sw2_inj &x = s->inj[k];
+            if (x.n == 0) continue;
+            if (x.n > s->d_cap[k]) {
+                HIPCHECK(hipStreamSynchronize(s->st));
+                if (s->d_idx[k]) HIPCHECK(hipFree(s->d_idx[k]));
+                if (s->d_inc[k]) HIPCHECK(hipFree(s->d_inc[k]));
+                if (s->d_trv[k]) HIPCHECK(hipFree(s->d_trv[k]));
+                s->d_idx[k] = NULL;
+                s->d_inc[k] = s->d_trv[k] = NULL;
+                s->d_cap[k] = 0;
+                const int cap = x.n + x.n / 2 + 1024;
+                HIPCHECK(hipMalloc((void **)&s->d_idx[k], sizeof(int) * (size_t)cap));
+                HIPCHECK(hipMalloc((void **)&s->d_inc[k], sizeof(double) * (size_t)cap));
+                HIPCHECK(hipMalloc((void **)&s->d_trv[k], sizeof(double) * (size_t)cap));
+                s->d_cap[k] = cap;
+            }
+            HIPCHECK(hipMemcpyAsync(s->d_idx[k], x.idx, sizeof(int) * (size_t)x.n, hipMemcpyHostToDevice, s->st));
+            HIPCHECK(hipMemcpyAsync(s->d_inc[k], x.inc, sizeof(double) * (size_t)x.n, hipMemcpyHostToDevice, s->st));
+            HIPCHECK(hipMemcpyAsync(s->d_trv[k], x.trv, sizeof(double) * (size_t)x.n, hipMemcpyHostToDevice, s->st));
+            k_inject<<<(x.n + 255) / 256, 256, 0, s->st>>>(s->fl, x.n, s->d_idx[k], s->d_inc[k], s->d_trv[k], x.has_inc, s->eta,
+                                                          s->tr, s->pert);
+            s->perturbs_dirty = 1;
+            if (x.has_inc) *eta_changed = 1;
         }
-        if (dalloc(&s->pinc, S) || dalloc(&s->ptr, S) || dalloc(&s->pdu, S) || dalloc(&s->pdv, S)) return -1;
-        HIPCHECK(hipMalloc((void **)&s->pmask, S));
     }
-    int K = sw2_gaussians_at(SW, s->Dt, t, 8, s->h_inc, s->h_trv, s->h_mask);
-    if (K < 0) return -1;
-    int V = sw2_vortices_at(SW, s->Dt, t, 8, s->h_dv[0], s->h_dv[1], s->h_dv[2]);
-    if (V < 0) return -1;
-    dim3 G = cells_grid(s->g), B = cells_block();
-    for (int k = 0; k < K; k++) {
-        HIPCHECK(hipMemcpyAsync(s->ptr, s->h_trv[k], sizeof(double) * S, hipMemcpyHostToDevice, s->st));
-        HIPCHECK(hipMemcpyAsync(s->pmask, s->h_mask[k], S, hipMemcpyHostToDevice, s->st));
-        k_assign_masked<<<(s->S + 255) / 256, 256, 0, s->st>>>(s->S, s->fl, s->tr, s->pert, s->ptr, s->pmask);
-        s->perturbs_dirty = 1;
-        if (!isnan(s->h_inc[k][0])) {
-            HIPCHECK(hipMemcpyAsync(s->pinc, s->h_inc[k], sizeof(double) * S, hipMemcpyHostToDevice, s->st));
+    if (SW->num_vortices) {
+        if (!s->h_dv[0][0]) {
+            for (int k = 0; k < 8; k++)
+                for (int q = 0; q < 3; q++) s->h_dv[q][k] = (double *)malloc(sizeof(double) * S);
+            if (dalloc(&s->pinc, S) || dalloc(&s->pdu, S) || dalloc(&s->pdv, S)) return -1;
+        }
+        int V = sw2_vortices_at(SW, s->Dt, t, 8, s->h_dv[0], s->h_dv[1], s->h_dv[2]);
+        if (V < 0) return -1;
+        dim3 G = cells_grid(s->g), B = cells_block();
+        for (int k = 0; k < V; k++) {
+            HIPCHECK(hipMemcpyAsync(s->pinc, s->h_dv[0][k], sizeof(double) * S, hipMemcpyHostToDevice, s->st));
+            HIPCHECK(hipMemcpyAsync(s->pdu, s->h_dv[1][k], sizeof(double) * S, hipMemcpyHostToDevice, s->st));
+            HIPCHECK(hipMemcpyAsync(s->pdv, s->h_dv[2][k], sizeof(double) * S, hipMemcpyHostToDevice, s->st));
             k_add<<<G, B, 0, s->st>>>(s->g, s->fl, s->eta, s->pinc);
+            k_add<<<G, B, 0, s->st>>>(s->g, s->fl, s->u, s->pdu);
+            k_add<<<G, B, 0, s->st>>>(s->g, s->fl, s->v, s->pdv);
+            launch_halo(s, s->eta);
+            launch_halo(s, s->u);
+            launch_halo(s, s->v);
             *eta_changed = 1;
+            HIPCHECK(hipStreamSynchronize(s->st));  // host scratch is reused
         }
-        HIPCHECK(hipStreamSynchronize(s->st));  // host scratch is reused
     }
-    for (int k = 0; k < V; k++) {
-        HIPCHECK(hipMemcpyAsync(s->pinc, s->h_dv[0][k], sizeof(double) * S, hipMemcpyHostToDevice, s->st));
-        HIPCHECK(hipMemcpyAsync(s->pdu, s->h_dv[1][k], sizeof(double) * S, hipMemcpyHostToDevice, s->st));
-        HIPCHECK(hipMemcpyAsync(s->pdv, s->h_dv[2][k], sizeof(double) * S, hipMemcpyHostToDevice, s->st));
-        k_add<<<G, B, 0, s->st>>>(s->g, s->fl, s->eta, s->pinc);
-        k_add<<<G, B, 0, s->st>>>(s->g, s->fl, s->u, s->pdu);
-        k_add<<<G, B, 0, s->st>>>(s->g, s->fl, s->v, s->pdv);
-        launch_halo(s, s->eta);
-        launch_halo(s, s->u);
-        launch_halo(s, s->v);
-        *eta_changed = 1;
-        HIPCHECK(hipStreamSynchronize(s->st));
-    }
+    // the lists are rebuilt next step: their copies must have left host memory
+    if (K) HIPCHECK(hipStreamSynchronize(s->st));
     (void)n;
     return 0;
 }
@@ -1276,7 +1300,7 @@ extern "C" void sw2_session_destroy(sw2_session *s) {
     if (!s) return;
     if (s->st) (void)hipStreamSynchronize(s->st);
     double *all[] = {s->d_tab, s->eta, s->u, s->v, s->eta2, s->u2, s->v2, s->tr, s->pert, s->wx, s->wy, s->duab, s->dvab, s->Lu, s->Lv, s->LLu, s->LLv,
-                     s->LLLu, s->LLLv, s->ze, s->di, s->le, s->lle, s->llle, s->psi, s->xi, s->pinc, s->ptr, s->pdu, s->pdv};
+                     s->LLLu, s->LLLv, s->ze, s->di, s->le, s->lle, s->llle, s->psi, s->xi, s->pinc, s->pdu, s->pdv};
     for (auto p : all)
         if (p) (void)hipFree(p);
     for (int k = 0; k < 3; k++) {
@@ -1285,7 +1309,6 @@ extern "C" void sw2_session_destroy(sw2_session *s) {
         if (s->Dv[k]) (void)hipFree(s->Dv[k]);
         if (s->Dtr[k]) (void)hipFree(s->Dtr[k]);
     }
-    if (s->pmask) (void)hipFree(s->pmask);
     float *fall[] = {s->m_eta, s->m_u, s->m_v, s->m_eta2, s->m_u2, s->m_v2};
     for (auto p : fall)
         if (p) (void)hipFree(p);
@@ -1301,11 +1324,12 @@ extern "C" void sw2_session_destroy(sw2_session *s) {
     if (s->fl) (void)hipFree(s->fl);
     if (s->st) (void)hipStreamDestroy(s->st);
     for (int k = 0; k < 8; k++) {
-        free(s->h_inc[k]);
-        free(s->h_trv[k]);
-        free(s->h_mask[k]);
         for (int q = 0; q < 3; q++) free(s->h_dv[q][k]);
+        if (s->d_idx[k]) (void)hipFree(s->d_idx[k]);
+        if (s->d_inc[k]) (void)hipFree(s->d_inc[k]);
+        if (s->d_trv[k]) (void)hipFree(s->d_trv[k]);
     }
+    sw2_inj_free(s->inj, 8);
     free(s);
 }
 

@@ -812,6 +812,243 @@ int sw2_gaussians_at(sw *SW, double Dt, double t, int maxk, double **inc, double
     return K;
 }
 
+/* ---- sparse injections: the cells inside the truncation radius only ---- */
+typedef struct {
+    int q, call;
+    double pf, tv;
+} inj_cell;
+
+static int inj_cmp(const void *a, const void *b) {
+    const inj_cell *x = (const inj_cell *)a, *y = (const inj_cell *)b;
+    if (x->q != y->q) return x->q < y->q ? -1 : 1;
+    return x->call < y->call ? -1 : (x->call > y->call);
+}
+
+typedef struct {
+    int i;
+    double c;
+} col_cos;
+
+static int col_cmp(const void *a, const void *b) {
+    const col_cos *x = (const col_cos *)a, *y = (const col_cos *)b;
+    if (x->c != y->c) return x->c > y->c ? -1 : 1;
+    return x->i - y->i;
+}
+
+/* the cells get_Gaussian_field (perturbs.c:348-395) accepts for one centre,
+ * appended to *cells: SW2's expressions, evaluated only where they can pass.
+ * In a row the distance falls as cos(lonc - lon) grows (cos(lat0)cos(lat1)
+ * >= 0 and every rounding is monotone), so the accepted columns are a
+ * prefix of the columns ordered by that cosine; the walk goes 8 columns
+ * past the last acceptance, and rows are candidates when their best column
+ * comes within a relative 1e-9 of the radius. */
+static int gaussian_cells(sw *SW, double lon, double lat, double amp, double sigma, double radius, double inject, int call,
+                          inj_cell **cells, int *n, int *cap, col_cos *cols) {
+    map *M = SW->M;
+    const int nx = SW->nx, ny = SW->ny;
+    const double Rm = (SW->rE + SW->rP) / 2.0, lat0 = g_lat2pclat(SW, lat);
+    for (int i = 1; i <= nx; i++) {
+        cols[i - 1].i = i;
+        cols[i - 1].c = cos(g_lonc(SW, i) - lon);
+    }
+    qsort(cols, (size_t)nx, sizeof(col_cos), col_cmp);
+    for (int j = 1; j <= ny; j++) {
+        const double lat1 = g_lat2pclat(SW, g_latc(SW, j));
+        const double a = sin(lat0) * sin(lat1), b = cos(lat0) * cos(lat1);
+        const double rbest = Rm * acos(a + b * cols[0].c);
+        if (!(rbest < radius * (1.0 + 1e-9))) continue;
+        int misses = 0;
+        for (int k = 0; k < nx && misses < 8; k++) {
+            const int i = cols[k].i;
+            double r = great_circle(lon, lat0, g_lonc(SW, i), lat1, Rm);
+            if (!(r < radius)) {
+                misses++;
+                continue;
+            }
+            misses = 0;
+            if (*n == *cap) {
+                int nc = *cap ? 2 * *cap : 4096;
+                inj_cell *nb = (inj_cell *)realloc(*cells, sizeof(inj_cell) * (size_t)nc);
+                if (!nb) return -1;
+                *cells = nb;
+                *cap = nc;
+            }
+            inj_cell *e = &(*cells)[(*n)++];
+            e->q = LP(i, j, M);
+            e->call = call;
+            e->pf = amp * exp(-(r * r) / (2.0 * (sigma * sigma)));
+            e->tv = (inject) * exp(-(r * r) / (2.0 * (sigma * sigma)));
+        }
+    }
+    return 0;
+}
+
+static int inj_reserve(sw2_inj *x, int n) {
+    if (n <= x->cap) return 0;
+    int nc = n + n / 2 + 1024;
+    int *ni = (int *)realloc(x->idx, sizeof(int) * (size_t)nc);
+    if (ni) x->idx = ni;
+    double *na = (double *)realloc(x->inc, sizeof(double) * (size_t)nc);
+    if (na) x->inc = na;
+    double *nt = (double *)realloc(x->trv, sizeof(double) * (size_t)nc);
+    if (nt) x->trv = nt;
+    if (!ni || !na || !nt) return -1;
+    x->cap = nc;
+    return 0;
+}
+
+void sw2_inj_free(sw2_inj *inj, int k) {
+    for (int q = 0; q < k; q++) {
+        free(inj[q].idx);
+        free(inj[q].inc);
+        free(inj[q].trv);
+        inj[q] = (sw2_inj){0, 0, 0, NULL, NULL, NULL};
+    }
+}
+
+int sw2_gaussians_sparse(sw *SW, double Dt, double t, int maxk, sw2_inj *inj) {
+    int K = 0;
+    col_cos *cols = (col_cos *)malloc(sizeof(col_cos) * (size_t)SW->nx);
+    inj_cell *cells = NULL;
+    int cap = 0, rc = 0;
+    if (!cols) {
+        sw2_set_error("out of host memory");
+        return -1;
+    }
+    for (int j = 0; j <= SW->num_Gaussians - 1 && !rc; j++) {
+        double *g = SW->Gaussians + j * N_COLS_GAUSS;
+        double t0g = g[0], t1g = g[1];
+        if (!((t0g <= t) && (t1g >= t))) continue;
+        if (K >= maxk) {
+            sw2_set_error("more than %d simultaneous injections", maxk);
+            rc = -1;
+            break;
+        }
+        /* the scalars of add_Gaussians / add_Gaussian_core, as in
+         * sw2_gaussians_at */
+        double tau = g[2];
+        if (tau < 0) {
+            sw2_set_error("Uhhh tau is %e and should be positive", tau);
+            rc = -1;
+            break;
+        }
+        double lon_g = DEG2RAD(g[3]), lat_g = DEG2RAD(g[4]);
+        int isdegvel = (int)g[5];
+        double vel = g[6];
+        if (isdegvel < 0 || isdegvel > 1) {
+            sw2_set_error("Perturbation table error: isdegvel should be 0 or 1");
+            rc = -1;
+            break;
+        }
+        double tt = (t - t0g), Om = 0;
+        if (isdegvel == 0) Om = vel / g_rZ(SW, lat_g);
+        if (isdegvel == 1) Om = DEG2RAD(vel);
+        double lon_g0 = lon_g;
+        lon_g = lon_g0 + Om * tt;
+        if (SW->prn) pprintf("add_Gaussians j=%d lon_Gauss=%f lat_Gauss = %f tt=%e \n", j, RAD2DEG(lon_g), RAD2DEG(lat_g), tt);
+        double Q = g[7];
+        int isdegsize = (int)g[8];
+        double sigma = g[9], radius = g[10], inject = g[11];
+        if (isdegsize < 0 || isdegsize > 1) {
+            sw2_set_error("Perturbation table error: isdegsize should be 0 or 1");
+            rc = -1;
+            break;
+        }
+        if (isdegsize == 1) {
+            sigma = sigma / 360.0 * 2 * PI_ * (SW->rE + SW->rP) / 2.0;
+            radius = radius / 360.0 * 2 * PI_ * (SW->rE + SW->rP) / 2.0;
+        }
+        double Qexp = 0, tauTH = 6 * tau;
+        double t_int = tau * log(exp((t1g - tauTH) / tau) + exp(t0g / tau));
+        if ((t < tauTH + t0g) && (t < t_int)) Qexp = Q * (1.0 - exp(-(t - t0g) / tau));
+        if ((t0g + tauTH <= t) && (t <= t1g - tauTH)) Qexp = Q;
+        if ((t > t1g - tauTH) && (t > t_int)) Qexp = Q * exp(-(t - (t1g - tauTH)) / tau);
+        double volume = Dt * Qexp, lon = lon_g, lat = lat_g;
+        if (SW->prn) pprintf("add_Gaussian_core lon=%e DEG lat=%e DEG volume=%e sigma=%e radius=%e \n", RAD2DEG(lon), RAD2DEG(lat), volume, sigma, radius);
+        while (lon < SW->lon0) lon += (SW->lon1 - SW->lon0);
+        while (lon > SW->lon1) lon -= (SW->lon1 - SW->lon0);
+        double amp = volume / (2.0 * PI_ * (sigma * sigma) * (1.0 - exp(-(radius * radius) / (2.0 * (sigma * sigma)))));
+        if (SW->prn) pprintf("add_Gaussian core lon=%.3f lat=%.3f volume=%.3e amplitude=%.3f\n", RAD2DEG(lon), RAD2DEG(lat), volume, amp);
+        int n = 0;
+        if (SW->prn) pprintf("get_Gaussian_field entering lon=%f DEG lat=%f DEG\n", RAD2DEG(lon), RAD2DEG(lat));
+        rc |= gaussian_cells(SW, lon, lat, amp, sigma, radius, inject, 0, &cells, &n, &cap, cols);
+        double lonRight = lon + (SW->lon1 - SW->lon0), lonLeft = lon - (SW->lon1 - SW->lon0);
+        if ((lon + radius / g_rZ(SW, lat)) >= SW->lon1) {
+            if (SW->prn) pprintf("get_Gaussian_field entering lon=%f DEG lat=%f DEG\n", RAD2DEG(lonLeft), RAD2DEG(lat));
+            rc |= gaussian_cells(SW, lonLeft, lat, amp, sigma, radius, inject, 1, &cells, &n, &cap, cols);
+        }
+        if ((lon - radius / g_rZ(SW, lat)) <= SW->lon0) {
+            if (SW->prn) pprintf("get_Gaussian_field entering lon=%f DEG lat=%f DEG\n", RAD2DEG(lonRight), RAD2DEG(lat));
+            rc |= gaussian_cells(SW, lonRight, lat, amp, sigma, radius, inject, 2, &cells, &n, &cap, cols);
+        }
+        if (rc) {
+            sw2_set_error("out of host memory");
+            break;
+        }
+        /* one entry per cell in SW2's row-major order; a cell two centres
+         * reach adds their values in call order, the tracer keeps the last */
+        qsort(cells, (size_t)n, sizeof(inj_cell), inj_cmp);
+        sw2_inj *x = &inj[K];
+        if (inj_reserve(x, n)) {
+            sw2_set_error("out of host memory");
+            rc = -1;
+            break;
+        }
+        int m = 0;
+        for (int a = 0; a < n;) {
+            double pf = 0.0, tv = 0.0;
+            int b = a;
+            for (; b < n && cells[b].q == cells[a].q; b++) {
+                pf += cells[b].pf;
+                tv = cells[b].tv;
+            }
+            x->idx[m] = cells[a].q;
+            x->inc[m] = pf;
+            x->trv[m] = tv;
+            m++;
+            a = b;
+        }
+        x->n = m;
+        /* the two volume sums over every interior cell in row-major order:
+         * cells outside the radius add an exact zero */
+        double V = 0.0;
+        for (int a = 0; a < m; a++) {
+            double s = SW->Dxc[x->idx[a]] * SW->Dyc[x->idx[a]];
+            double v = x->inc[a] * s;
+            V += v;
+        }
+        if (volume == 0.0) pprintf("Volume to be injected is currently 0.\n");
+        if (fabs(volume) > 1e-7) {
+            if (V == 0.0) {
+                sw2_set_error("Global total injected volume Vg=%e equals 0", V);
+                rc = -1;
+                break;
+            }
+            double rf = (volume / V), Vcheck = 0.0;
+            if (SW->prn) pprintf("Perturbation reescale factor is %e\n", rf);
+            for (int a = 0; a < m; a++) {
+                double v0 = rf * x->inc[a];
+                x->inc[a] = v0;
+                double s = SW->Dxc[x->idx[a]] * SW->Dyc[x->idx[a]];
+                double v = v0 * s;
+                Vcheck += v;
+            }
+            if (fabs(Vcheck - volume) / volume > 1e-1) {
+                sw2_set_error("Total injected volume Vcheck=%e does not match input volume=%e (difference is=%e)", Vcheck, volume, fabs(Vcheck - volume));
+                rc = -1;
+                break;
+            }
+            x->has_inc = 1;
+        } else {
+            x->has_inc = 0;
+        }
+        K++;
+    }
+    free(cols);
+    free(cells);
+    return rc ? -1 : K;
+}
+
 int sw2_vortices_at(sw *SW, double Dt, double t, int maxk, double **deta, double **du, double **dv) {
     int K = 0;
     map *M = SW->M;

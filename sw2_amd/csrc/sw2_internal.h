@@ -53,6 +53,22 @@ void sw2_cfl_rest(sw *SW, double *max_x, double *max_y);
  * Mirrors add_Gaussians / add_Gaussian_core / get_Gaussian_field
  * (perturbs.c:22-395).  Returns -1 on SW2's fatal conditions. */
 int sw2_gaussians_at(sw *SW, double Dt, double t, int maxk, double **inc, double **trval, unsigned char **mask);
+/* The same injections as sparse lists: for injection k, the cells inside
+ * its truncation radius in SW2's row-major order (inj[k].idx, field
+ * offsets), the eta increment there (inj[k].inc; has_inc 0 when SW2 leaves
+ * eta alone) and the tracer value (inj[k].trv).  Only those cells are
+ * evaluated: per row the accepted columns are the ones whose cos(dlon) is
+ * largest (the distance is monotone in it), found by walking the columns
+ * in that order; every value, and both volume sums, are SW2's bit for bit.
+ * The arrays grow as needed and stay owned by inj (sw2_inj_free). */
+typedef struct {
+    int n, cap, has_inc;
+    int *idx;
+    double *inc, *trv;
+} sw2_inj;
+int sw2_gaussians_sparse(sw *SW, double Dt, double t, int maxk, sw2_inj *inj);
+void sw2_inj_free(sw2_inj *inj, int k);
+
 /* Vortex increments at time t (perturbs.c:397-460): returns the number of
  * vortices starting in (t-Dt, t]; increments are summed per vortex k in
  * eta/u/v order into deta[k], du[k], dv[k]. */

@@ -107,6 +107,20 @@ def test_storm_injection_with_tracer_and_vortex():
     assert_same(w, o, "injection+vortex")
 
 
+def test_storm_injections_across_the_seam_and_overlapping():
+    """Two tracer-injecting storms near longitude 0: one centred on the
+    periodic seam (its wrap copies inject on both sides), one moving west
+    across it; their fields overlap.  The library evaluates only the cells
+    inside each radius; every value must be the oracle's full-grid one."""
+    G = np.array([[0.0, 2000.0, 100.0, 359.0, 10.0, 1, 0.5, 2e11, 1, 4.0, 12.0, 1.0],
+                  [0.0, 2000.0, 150.0, 5.0, 14.0, 0, -30.0, 1e11, 1, 3.0, 9.0, 0.5]])
+    p = cases.scaled_params(128, 64, steps=30, lat=70.0, Gaussians=G)
+    w, o, s, fs, fo = run_both(p, cases.single_bump(8), 30)
+    assert fs == fo == 0
+    assert np.abs(w.field("tracer")).max() > 0.1
+    assert_same(w, o, "two storms at the seam")
+
+
 def test_jupiter_energy_case(golden):
     """The reference's own JupiterEnergyTest case (SUPERBEE, drag, moving
     storm injection from t0) through the first 400 steps."""
