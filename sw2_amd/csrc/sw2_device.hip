@@ -517,18 +517,25 @@ __global__ void k_add(Grid g, Flags *fl, double *f, const double *inc) {
 // one storm injection from its sparse list (the cells inside the radius,
 // perturbs.c:348-395): the tracer value and the perturbation mark there, and
 // the eta increment when the injection changes eta
-__global__ void k_inject(Flags *fl, int n, const int *idx, const double *inc, const double *trv, int has_inc, double *eta,
-                         double *tr, double *pert) {
+// SW2 injects before the step's checks (core.c:280-312): a check failing in
+// this step (the fused step checks u and v before the injection) leaves it
+// to run; an earlier failure or this step's CFL guard does not
+__global__ void k_inject(Flags *fl, int step, int n, const int *idx, const double *inc, const double *trv, int has_inc,
+                         double *eta, double *tr, double *pert) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (fl->fail_step || k >= n) return;
+    if ((fl->fail_step && (fl->fail_step != step || fl->fail_kind == 1)) || k >= n) return;
     const int q = idx[k];
     tr[q] = trv[k];
     pert[q] = 1.0;
     if (has_inc) eta[q] += inc[k];
 }
 
-__global__ void k_zero(int S, double *f) {
+// clear the perturbation marks of the previous injections at step `step`;
+// like the injection itself, not once SW2 has stopped (an earlier failure,
+// or this step's CFL guard), so the marks of the failing step remain
+__global__ void k_zero_pert(Flags *fl, int step, int S, double *f) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (fl->fail_step && (fl->fail_step != step || fl->fail_kind == 1)) return;
     if (q < S) f[q] = 0;
 }
 
@@ -617,6 +624,13 @@ struct sw2_session {
     int *d_idx[8], d_cap[8];
     double *d_inc[8], *d_trv[8];
     double *h_dv[3][8];
+    // pinned staging of the lists, two sets used alternately: a set is
+    // refilled once the event after its copies has passed, so the host
+    // builds the next step's lists while the device still runs this one
+    char *h_stage[2];
+    size_t stage_cap[2];
+    hipEvent_t stage_ev[2];
+    int stage_i;
 };
 
 static int dalloc(double **p, size_t n) {
@@ -767,11 +781,10 @@ extern "C" int sw2_session_create(sw *SW, int form, sw2_session **out) {
         memcpy(&b, &SW->windy[q], sizeof b);
         if (b) windy_zero = 0;
     }
-    s->fused = !force_phased && tables_ok && wind_rows_ok && windy_zero && !s->tracer_active && SW->num_Gaussians == 0 &&
-               SW->num_vortices == 0 && SW->nu6 == 0 && SW->nx >= 2 * FTX && SW->ny >= 4 &&
-               !(SW->nu2 != 0 && SW->nu4 == 0);
+    s->fused = !force_phased && tables_ok && wind_rows_ok && windy_zero && SW->num_vortices == 0 && SW->nu6 == 0 &&
+               SW->nx >= 2 * FTX && SW->ny >= 4 && !(SW->nu2 != 0 && SW->nu4 == 0);
     s->hvmode = s->hv ? 2 : 0;
-    if (form == SW2_FORM_MIXED && !s->fused) {
+    if (form == SW2_FORM_MIXED && !(s->fused && !s->tracer_active && SW->num_Gaussians == 0)) {
         sw2_set_error("the mixed-precision form needs a case the fused step takes: no in-loop storms or vortices, "
                       "no del-6, a zero tracer, a j-only wind, nx >= %d, ny >= 4", 2 * FTX);
         free(tab);
@@ -925,9 +938,34 @@ static int host_perturbations(sw2_session *s, int n, double t, int *eta_changed)
         // step's kernels; only those cells travel
         K = sw2_gaussians_sparse(SW, s->Dt, t, 8, s->inj);
         if (K < 0) return -1;
+        const int b = s->stage_i;
+        s->stage_i ^= 1;
+        size_t need = 0;
+        for (int k = 0; k < K; k++) need += (size_t)s->inj[k].n * (sizeof(int) + 2 * sizeof(double)) + 64;
+        if (s->stage_ev[b]) HIPCHECK(hipEventSynchronize(s->stage_ev[b]));
+        else HIPCHECK(hipEventCreateWithFlags(&s->stage_ev[b], hipEventDisableTiming));
+        if (need > s->stage_cap[b]) {
+            if (s->h_stage[b]) HIPCHECK(hipHostFree(s->h_stage[b]));
+            s->h_stage[b] = NULL;
+            s->stage_cap[b] = 0;
+            const size_t cap = need + need / 2 + (1 << 16);
+            HIPCHECK(hipHostMalloc((void **)&s->h_stage[b], cap, hipHostMallocDefault));
+            s->stage_cap[b] = cap;
+        }
+        char *stage = s->h_stage[b];
+        size_t off = 0;
         for (int k = 0; k < K; k++) {
             const sw2_inj &x = s->inj[k];
             if (x.n == 0) continue;
+            int *h_idx = (int *)(stage + off);
+            memcpy(h_idx, x.idx, sizeof(int) * (size_t)x.n);
+            off += (sizeof(int) * (size_t)x.n + 7) & ~(size_t)7;
+            double *h_inc = (double *)(stage + off);
+            memcpy(h_inc, x.inc, sizeof(double) * (size_t)x.n);
+            off += sizeof(double) * (size_t)x.n;
+            double *h_trv = (double *)(stage + off);
+            memcpy(h_trv, x.trv, sizeof(double) * (size_t)x.n);
+            off += sizeof(double) * (size_t)x.n;
             if (x.n > s->d_cap[k]) {
                 HIPCHECK(hipStreamSynchronize(s->st));
                 if (s->d_idx[k]) HIPCHECK(hipFree(s->d_idx[k]));
@@ -942,14 +980,15 @@ static int host_perturbations(sw2_session *s, int n, double t, int *eta_changed)
                 HIPCHECK(hipMalloc((void **)&s->d_trv[k], sizeof(double) * (size_t)cap));
                 s->d_cap[k] = cap;
             }
-            HIPCHECK(hipMemcpyAsync(s->d_idx[k], x.idx, sizeof(int) * (size_t)x.n, hipMemcpyHostToDevice, s->st));
-            HIPCHECK(hipMemcpyAsync(s->d_inc[k], x.inc, sizeof(double) * (size_t)x.n, hipMemcpyHostToDevice, s->st));
-            HIPCHECK(hipMemcpyAsync(s->d_trv[k], x.trv, sizeof(double) * (size_t)x.n, hipMemcpyHostToDevice, s->st));
-            k_inject<<<(x.n + 255) / 256, 256, 0, s->st>>>(s->fl, x.n, s->d_idx[k], s->d_inc[k], s->d_trv[k], x.has_inc, s->eta,
-                                                          s->tr, s->pert);
+            HIPCHECK(hipMemcpyAsync(s->d_idx[k], h_idx, sizeof(int) * (size_t)x.n, hipMemcpyHostToDevice, s->st));
+            HIPCHECK(hipMemcpyAsync(s->d_inc[k], h_inc, sizeof(double) * (size_t)x.n, hipMemcpyHostToDevice, s->st));
+            HIPCHECK(hipMemcpyAsync(s->d_trv[k], h_trv, sizeof(double) * (size_t)x.n, hipMemcpyHostToDevice, s->st));
+            k_inject<<<(x.n + 255) / 256, 256, 0, s->st>>>(s->fl, n, x.n, s->d_idx[k], s->d_inc[k], s->d_trv[k], x.has_inc,
+                                                          s->eta, s->tr, s->pert);
             s->perturbs_dirty = 1;
             if (x.has_inc) *eta_changed = 1;
         }
+        HIPCHECK(hipEventRecord(s->stage_ev[b], s->st));
     }
     if (SW->num_vortices) {
         if (!s->h_dv[0][0]) {
@@ -974,9 +1013,6 @@ static int host_perturbations(sw2_session *s, int n, double t, int *eta_changed)
             HIPCHECK(hipStreamSynchronize(s->st));  // host scratch is reused
         }
     }
-    // the lists are rebuilt next step: their copies must have left host memory
-    if (K) HIPCHECK(hipStreamSynchronize(s->st));
-    (void)n;
     return 0;
 }
 
@@ -984,13 +1020,13 @@ template <int SCH, int HV>
 static void launch_fused(sw2_session *s, int n, int k, int slot) {
     dim3 G((s->g.nx + FTX - 1) / FTX, (s->g.ny + FTY - 1) / FTY);
     EtaArgs ea{s->eta, s->u, s->v, s->Deta[1], s->Deta[2], s->tr, s->eta2, s->Deta[0], s->Dtr[0], n, k, slot,
-               s->Dt, s->depth, 2.0 * s->Dt * (s->sigma), 2.0 * s->Dt * (s->tracer_dis), s->c2, s->c4, s->courant,
-               s->rest_x, s->rest_y};
+               s->tracer_active, s->Dt, s->depth, 2.0 * s->Dt * (s->sigma), 2.0 * s->Dt * (s->tracer_dis), s->c2, s->c4,
+               s->courant, s->rest_x, s->rest_y};
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i], s->st);
     kf_eta<SCH, HV><<<G, NTE, 0, s->st>>>(s->g, s->fl, ea);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 1], s->st);
     UVArgs ua{s->eta2, s->u, s->v, s->Du[1], s->Du[2], s->Dv[1], s->Dv[2], s->Dtr[0], s->Dtr[1], s->Dtr[2],
-              s->u2, s->v2, s->Du[0], s->Dv[0], s->tr, n, k, slot ^ 1, s->Dt, s->sigma, s->c2, s->c4};
+              s->u2, s->v2, s->Du[0], s->Dv[0], s->tr, n, k, slot ^ 1, s->SW->num_Gaussians > 0, s->Dt, s->sigma, s->c2, s->c4};
     kf_uv<SCH, HV><<<G, NTF, 0, s->st>>>(s->g, s->fl, ua);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 2], s->st);
 }
@@ -1019,8 +1055,8 @@ static int step_fused(sw2_session *s) {
         k_cfl<<<G, B, 0, s->st>>>(s->g, p, s->fl, slot);
         s->cfl_ready = 1;
     }
-    // two kernels; the tracer (identically zero until u or v goes
-    // non-finite) is advanced inside them once that happens
+    // two kernels; the tracer is advanced inside them when it can be
+    // nonzero (otherwise once u or v goes non-finite)
     SCH_SWITCH(launch_fused_hv, s, n, k, slot);
     swap_ptr(&s->eta, &s->eta2);
     swap_ptr(&s->u, &s->u2);
@@ -1030,8 +1066,18 @@ static int step_fused(sw2_session *s) {
     rot(s->Dv);
     rot(s->Dtr);
     if (s->perturbs_dirty) {
-        k_zero<<<(s->S + 255) / 256, 256, 0, s->st>>>(s->S, s->pert);
+        k_zero_pert<<<(s->S + 255) / 256, 256, 0, s->st>>>(s->fl, n, s->S, s->pert);
         s->perturbs_dirty = 0;
+    }
+    if (s->SW->num_Gaussians) {
+        // storms after u and v (core.c:280-300), their halo images, then the
+        // checks of tracer and eta that the kernels left for after them
+        int eta_changed = 0;
+        if (host_perturbations(s, n, t, &eta_changed)) return -1;
+        launch_halo(s, s->tr, !s->tracer_active);
+        if (eta_changed) launch_halo(s, s->eta);
+        k_check<<<G, B, 0, s->st>>>(s->g, s->fl, s->tr, n, 2, !s->tracer_active);
+        k_check<<<G, B, 0, s->st>>>(s->g, s->fl, s->eta, n, 3, 0);
     }
     s->n = n;
     s->t = t;
@@ -1131,7 +1177,7 @@ static int step_phased(sw2_session *s) {
     launch_halo(s, s->u);
     launch_halo(s, s->v);
     if (s->perturbs_dirty) {
-        k_zero<<<(s->S + 255) / 256, 256, 0, s->st>>>(s->S, s->pert);
+        k_zero_pert<<<(s->S + 255) / 256, 256, 0, s->st>>>(s->fl, n, s->S, s->pert);
         s->perturbs_dirty = 0;
     }
     int eta_changed = 0;
@@ -1330,6 +1376,10 @@ extern "C" void sw2_session_destroy(sw2_session *s) {
         if (s->d_trv[k]) (void)hipFree(s->d_trv[k]);
     }
     sw2_inj_free(s->inj, 8);
+    for (int b = 0; b < 2; b++) {
+        if (s->h_stage[b]) (void)hipHostFree(s->h_stage[b]);
+        if (s->stage_ev[b]) (void)hipEventDestroy(s->stage_ev[b]);
+    }
     free(s);
 }
 

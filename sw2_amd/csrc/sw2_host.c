@@ -829,6 +829,21 @@ typedef struct {
     double c;
 } col_cos;
 
+/* cells of one row by field offset (insertion sort for the usual short
+ * rows) */
+static void sort_row(inj_cell *c, int n) {
+    if (n > 64) {
+        qsort(c, (size_t)n, sizeof(inj_cell), inj_cmp);
+        return;
+    }
+    for (int a = 1; a < n; a++) {
+        inj_cell x = c[a];
+        int b = a - 1;
+        for (; b >= 0 && c[b].q > x.q; b--) c[b + 1] = c[b];
+        c[b + 1] = x;
+    }
+}
+
 static int col_cmp(const void *a, const void *b) {
     const col_cos *x = (const col_cos *)a, *y = (const col_cos *)b;
     if (x->c != y->c) return x->c > y->c ? -1 : 1;
@@ -852,15 +867,46 @@ static int gaussian_cells(sw *SW, double lon, double lat, double amp, double sig
         cols[i - 1].c = cos(g_lonc(SW, i) - lon);
     }
     qsort(cols, (size_t)nx, sizeof(col_cos), col_cmp);
-    for (int j = 1; j <= ny; j++) {
+    /* Rows whose (parametric) latitude lies farther than the radius from
+     * the centre's hold no accepted cell: the central angle between two
+     * points is at least their latitude difference, and the computed
+     * distance is within a relative 1e-6 of the true one once the angle
+     * exceeds 1e-4 rad (the error of acos near 1 grows as 1/angle).  The
+     * latitude rises with j, so the band is found by bisection; tinier
+     * radii scan every row. */
+    int jlo = 1, jhi = ny;
+    const double band = radius / Rm;
+    if (band >= 1e-4) {
+        const double lim = band * (1.0 + 1e-6);
+        int lo = 1, hi = ny + 1; /* first row with lat1 >= lat0 - lim */
+        while (lo < hi) {
+            int mid = (lo + hi) / 2;
+            if (g_lat2pclat(SW, g_latc(SW, mid)) >= lat0 - lim) hi = mid;
+            else lo = mid + 1;
+        }
+        jlo = lo;
+        lo = 1, hi = ny + 1; /* first row with lat1 > lat0 + lim */
+        while (lo < hi) {
+            int mid = (lo + hi) / 2;
+            if (g_lat2pclat(SW, g_latc(SW, mid)) > lat0 + lim) hi = mid;
+            else lo = mid + 1;
+        }
+        jhi = lo - 1;
+    }
+    for (int j = jlo; j <= jhi; j++) {
         const double lat1 = g_lat2pclat(SW, g_latc(SW, j));
         const double a = sin(lat0) * sin(lat1), b = cos(lat0) * cos(lat1);
         const double rbest = Rm * acos(a + b * cols[0].c);
         if (!(rbest < radius * (1.0 + 1e-9))) continue;
+        const int row0 = *n;
         int misses = 0;
         for (int k = 0; k < nx && misses < 8; k++) {
             const int i = cols[k].i;
-            double r = great_circle(lon, lat0, g_lonc(SW, i), lat1, Rm);
+            /* great_circle(lon, lat0, lonc(i), lat1, Rm) term by term: its
+             * two latitude products are the row's a and b, and its
+             * cos(lonc(i) - lon) is cols[k].c, the same call on the same
+             * operands */
+            double r = Rm * acos(a + b * cols[k].c);
             if (!(r < radius)) {
                 misses++;
                 continue;
@@ -876,11 +922,30 @@ static int gaussian_cells(sw *SW, double lon, double lat, double amp, double sig
             inj_cell *e = &(*cells)[(*n)++];
             e->q = LP(i, j, M);
             e->call = call;
-            e->pf = amp * exp(-(r * r) / (2.0 * (sigma * sigma)));
-            e->tv = (inject) * exp(-(r * r) / (2.0 * (sigma * sigma)));
+            /* SW2 evaluates this exp twice, for eta and for the tracer:
+             * one call gives both values */
+            const double ex = exp(-(r * r) / (2.0 * (sigma * sigma)));
+            e->pf = amp * ex;
+            e->tv = (inject) * ex;
         }
+        sort_row(*cells + row0, *n - row0);
     }
     return 0;
+}
+
+/* merge the cell lists of the calls (each in field order, calls in order)
+ * into field order, a cell of several calls in call order */
+static int merge_calls(const inj_cell *c, const int *end, int ncall, inj_cell *out) {
+    int at[3] = {0, end[0], end[1]}, m = 0;
+    for (;;) {
+        int best = -1;
+        for (int k = 0; k < ncall; k++) {
+            if (at[k] >= end[k]) continue;
+            if (best < 0 || c[at[k]].q < c[at[best]].q) best = k;
+        }
+        if (best < 0) return m;
+        out[m++] = c[at[best]++];
+    }
 }
 
 static int inj_reserve(sw2_inj *x, int n) {
@@ -909,8 +974,8 @@ void sw2_inj_free(sw2_inj *inj, int k) {
 int sw2_gaussians_sparse(sw *SW, double Dt, double t, int maxk, sw2_inj *inj) {
     int K = 0;
     col_cos *cols = (col_cos *)malloc(sizeof(col_cos) * (size_t)SW->nx);
-    inj_cell *cells = NULL;
-    int cap = 0, rc = 0;
+    inj_cell *cells = NULL, *merged = NULL;
+    int cap = 0, mcap = 0, rc = 0;
     if (!cols) {
         sw2_set_error("out of host memory");
         return -1;
@@ -969,25 +1034,50 @@ int sw2_gaussians_sparse(sw *SW, double Dt, double t, int maxk, sw2_inj *inj) {
         while (lon > SW->lon1) lon -= (SW->lon1 - SW->lon0);
         double amp = volume / (2.0 * PI_ * (sigma * sigma) * (1.0 - exp(-(radius * radius) / (2.0 * (sigma * sigma)))));
         if (SW->prn) pprintf("add_Gaussian core lon=%.3f lat=%.3f volume=%.3e amplitude=%.3f\n", RAD2DEG(lon), RAD2DEG(lat), volume, amp);
-        int n = 0;
+        int n = 0, end[3], ncall = 0;
         if (SW->prn) pprintf("get_Gaussian_field entering lon=%f DEG lat=%f DEG\n", RAD2DEG(lon), RAD2DEG(lat));
         rc |= gaussian_cells(SW, lon, lat, amp, sigma, radius, inject, 0, &cells, &n, &cap, cols);
+        end[ncall++] = n;
         double lonRight = lon + (SW->lon1 - SW->lon0), lonLeft = lon - (SW->lon1 - SW->lon0);
         if ((lon + radius / g_rZ(SW, lat)) >= SW->lon1) {
             if (SW->prn) pprintf("get_Gaussian_field entering lon=%f DEG lat=%f DEG\n", RAD2DEG(lonLeft), RAD2DEG(lat));
             rc |= gaussian_cells(SW, lonLeft, lat, amp, sigma, radius, inject, 1, &cells, &n, &cap, cols);
+            end[ncall++] = n;
         }
         if ((lon - radius / g_rZ(SW, lat)) <= SW->lon0) {
             if (SW->prn) pprintf("get_Gaussian_field entering lon=%f DEG lat=%f DEG\n", RAD2DEG(lonRight), RAD2DEG(lat));
             rc |= gaussian_cells(SW, lonRight, lat, amp, sigma, radius, inject, 2, &cells, &n, &cap, cols);
+            end[ncall++] = n;
+        }
+        if (!rc && n > mcap) {
+            inj_cell *nb = (inj_cell *)realloc(merged, sizeof(inj_cell) * (size_t)n);
+            if (nb) {
+                merged = nb;
+                mcap = n;
+            } else
+                rc = -1;
         }
         if (rc) {
             sw2_set_error("out of host memory");
             break;
         }
         /* one entry per cell in SW2's row-major order; a cell two centres
-         * reach adds their values in call order, the tracer keeps the last */
-        qsort(cells, (size_t)n, sizeof(inj_cell), inj_cmp);
+         * reach adds their values in call order, the tracer keeps the last.
+         * Each call's cells are in field order already (rows in order, each
+         * row sorted), so a merge orders them. */
+        if (merge_calls(cells, end, ncall, merged) != n) {
+            sw2_set_error("injection cell merge lost cells");
+            rc = -1;
+            break;
+        }
+        {
+            inj_cell *tmp = cells;
+            int tcap = cap;
+            cells = merged;
+            cap = mcap;
+            merged = tmp;
+            mcap = tcap;
+        }
         sw2_inj *x = &inj[K];
         if (inj_reserve(x, n)) {
             sw2_set_error("out of host memory");
@@ -1046,6 +1136,7 @@ int sw2_gaussians_sparse(sw *SW, double Dt, double t, int maxk, sw2_inj *inj) {
     }
     free(cols);
     free(cells);
+    free(merged);
     return rc ? -1 : K;
 }
 

@@ -125,3 +125,10 @@ def test_mixed_refuses_cases_the_fused_step_does_not_take():
     w, _ = cases.make_world(p, cases.single_bump(1))
     with pytest.raises(SW2Error, match="mixed-precision form"):
         Session(w, FORM_MIXED)
+    # storms run on the fused FP64 step, not in the mixed form
+    G = np.array([[0.0, 3000.0, 100.0, 100.0, 10.0, 1, 1.0, 2e11, 1, 4.0, 10.0, 1.0]])
+    p = cases.scaled_params(128, 32, steps=5, lat=60.0, Gaussians=G)
+    w, _ = cases.make_world(p, cases.single_bump(1))
+    assert Session(w).fused
+    with pytest.raises(SW2Error, match="mixed-precision form"):
+        Session(w, FORM_MIXED)

@@ -115,10 +115,65 @@ def test_storm_injections_across_the_seam_and_overlapping():
     G = np.array([[0.0, 2000.0, 100.0, 359.0, 10.0, 1, 0.5, 2e11, 1, 4.0, 12.0, 1.0],
                   [0.0, 2000.0, 150.0, 5.0, 14.0, 0, -30.0, 1e11, 1, 3.0, 9.0, 0.5]])
     p = cases.scaled_params(128, 64, steps=30, lat=70.0, Gaussians=G)
-    w, o, s, fs, fo = run_both(p, cases.single_bump(8), 30)
-    assert fs == fo == 0
-    assert np.abs(w.field("tracer")).max() > 0.1
-    assert_same(w, o, "two storms at the seam")
+    for form in (0, PHASED):
+        w, o, s, fs, fo = run_both(p, cases.single_bump(8), 30, form=form)
+        assert s.fused == (form == 0)
+        assert fs == fo == 0
+        assert np.abs(w.field("tracer")).max() > 0.1
+        assert_same(w, o, f"two storms at the seam, {'fused' if s.fused else 'phased'}")
+
+
+@pytest.mark.parametrize("sch", [2, 3, 5])
+def test_fused_storms_with_a_moving_tracer(sch):
+    """The fused step with storms: the tracer they inject (and a nonzero
+    tracer from the start) advected by BODYTRACER every step, the
+    injections applied after u and v, and the tracer and eta checks after
+    the injections, as SW2 orders them.  Under van Leer the symmetric storm
+    bump gives a face ratio of exactly -1, whose 0/0 is NaN in SW2; the NaN
+    reaches cell (1,1) and the v check stops both runs on the same step,
+    with the same frozen state."""
+    G = np.array([[0.0, 3000.0, 100.0, 200.0, 30.0, 1, 1.5, 2e11, 1, 4.0, 10.0, 1.0],
+                  [600.0, 1800.0, 50.0, 60.0, -40.0, 0, 40.0, -1e11, 1, 3.0, 7.0, 0.0]])
+    p = cases.scaled_params(256, 96, steps=60, lat=75.0, sch=sch, Gaussians=G)
+    w, (eta, u, v) = cases.make_world(p, cases.single_bump(9))
+    rng = np.random.default_rng(sch)
+    tr0 = rng.uniform(0.0, 0.5, size=w.field("tracer").shape)
+    w.field("tracer")[:] = tr0
+    o = Oracle(p)
+    o.load(eta, u, v)
+    o.field("tracer")[:] = tr0
+    s = Session(w)
+    assert s.fused
+    s.begin(0, p.t0)
+    s.step(60)
+    fs, _ = s.failed_step()
+    s.download()
+    fo, _ = o.step(60)
+    assert fs == fo
+    if sch == 2:
+        assert fo == 0
+    assert_same(w, o, f"fused storms sch={sch}")
+
+
+def test_fused_storm_guards():
+    """A check failing in a storm step still sees that step's injection
+    (SW2 injects before it checks); a CFL failure stops the step before
+    the injection."""
+    G = np.array([[0.0, 3000.0, 100.0, 100.0, 10.0, 1, 1.0, 2e11, 1, 4.0, 10.0, 1.0]])
+    p = cases.scaled_params(128, 20, steps=5, lat=70.0, CFL=1e9, Gaussians=G)
+    for form in (0, PHASED):
+        # steps past the failure leave the frozen state (perturbation marks
+        # included) as SW2 left it
+        w, o, s, fs, fo = run_both(p, [(100.0, 10.0, 5e10, 5.0)], 5, form=form)
+        assert s.fused == (form == 0) and fo == 1 and fs == 1
+        assert "eta" in s.failed_step()[1]
+        assert np.abs(w.field("perturbs")).max() == 1.0
+        assert_same(w, o, f"storm frozen at the stats failure, form {form}")
+    p = cases.scaled_params(128, 20, steps=10, lat=70.0, Dt=50000.0, Gaussians=G)
+    for form in (0, PHASED):
+        w, o, s, fs, fo = run_both(p, cases.single_bump(7), 10, form=form)
+        assert s.fused == (form == 0) and fo > 0 and fs == fo
+        assert_same(w, o, f"storm frozen at the CFL failure, form {form}")
 
 
 def test_jupiter_energy_case(golden):
