@@ -20,6 +20,9 @@
 #include <string.h>
 #include <sys/stat.h>
 #include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "sw2_internal.h"
 
 /* ------------------------------------------------------------------ errors */
@@ -829,25 +832,144 @@ typedef struct {
     double c;
 } col_cos;
 
-/* cells of one row by field offset (insertion sort for the usual short
- * rows) */
-static void sort_row(inj_cell *c, int n) {
-    if (n > 64) {
+/* cells of one row (one call: distinct columns) by field offset.  Short
+ * rows: insertion sort.  Longer ones: each cell goes to the slot of its
+ * offset within the row's span (slot: scratch of nslot ints, all -1 on entry
+ * and on return), read back in order; a span too wide for that is sorted. */
+static void sort_row(inj_cell *c, int n, int *slot, int nslot, inj_cell *tmp) {
+    if (n <= 16) {
+        for (int a = 1; a < n; a++) {
+            inj_cell x = c[a];
+            int b = a - 1;
+            for (; b >= 0 && c[b].q > x.q; b--) c[b + 1] = c[b];
+            c[b + 1] = x;
+        }
+        return;
+    }
+    int lo = c[0].q, hi = c[0].q;
+    for (int a = 1; a < n; a++) {
+        if (c[a].q < lo) lo = c[a].q;
+        if (c[a].q > hi) hi = c[a].q;
+    }
+    const int span = hi - lo + 1;
+    if (span > nslot || span > 8 * n) {
         qsort(c, (size_t)n, sizeof(inj_cell), inj_cmp);
         return;
     }
-    for (int a = 1; a < n; a++) {
-        inj_cell x = c[a];
-        int b = a - 1;
-        for (; b >= 0 && c[b].q > x.q; b--) c[b + 1] = c[b];
-        c[b + 1] = x;
-    }
+    for (int a = 0; a < n; a++) slot[c[a].q - lo] = a;
+    int m = 0;
+    for (int k = 0; k < span; k++)
+        if (slot[k] >= 0) {
+            tmp[m++] = c[slot[k]];
+            slot[k] = -1;
+        }
+    memcpy(c, tmp, sizeof(inj_cell) * (size_t)n);
 }
 
 static int col_cmp(const void *a, const void *b) {
     const col_cos *x = (const col_cos *)a, *y = (const col_cos *)b;
     if (x->c != y->c) return x->c > y->c ? -1 : 1;
     return x->i - y->i;
+}
+
+/* the accepted cells of rows jlo..jhi for one centre (see gaussian_cells),
+ * appended to *cells in row order, each row sorted by column */
+static int band_cells(sw *SW, int jlo, int jhi, double lat0, double Rm, double amp, double sigma, double radius,
+                      double inject, int call, inj_cell **cells, int *n, int *cap, const col_cos *cols, int ntrust) {
+    map *M = SW->M;
+    const int nslot = SW->nx + 4;
+    int *slot = (int *)malloc(sizeof(int) * (size_t)nslot);
+    inj_cell *tmp = (inj_cell *)malloc(sizeof(inj_cell) * (size_t)nslot);
+    if (!slot || !tmp) {
+        free(slot);
+        free(tmp);
+        return -1;
+    }
+    for (int k = 0; k < nslot; k++) slot[k] = -1;
+    int rc = 0;
+    /* the list in locals: threads keep their counters in registers, not in
+     * a cache line they share */
+    inj_cell *lc = *cells;
+    int ln = *n, lcap = *cap;
+    for (int j = jlo; j <= jhi && rc == 0; j++) {
+        const double lat1 = g_lat2pclat(SW, g_latc(SW, j));
+        const double a = sin(lat0) * sin(lat1), b = cos(lat0) * cos(lat1);
+        const double rbest = Rm * acos(a + b * cols[0].c);
+        if (!(rbest < radius * (1.0 + 1e-9))) continue;
+        const int row0 = ln;
+        int misses = 0;
+        for (int k = 0; misses < 8; k++) {
+            if (k >= ntrust) {
+                if (ntrust < SW->nx) rc = 1; /* the walk needs columns past the sorted window */
+                break;
+            }
+            const int i = cols[k].i;
+            /* great_circle(lon, lat0, lonc(i), lat1, Rm) term by term: its
+             * two latitude products are the row's a and b, and its
+             * cos(lonc(i) - lon) is cols[k].c, the same call on the same
+             * operands */
+            double r = Rm * acos(a + b * cols[k].c);
+            if (!(r < radius)) {
+                misses++;
+                continue;
+            }
+            misses = 0;
+            if (ln == lcap) {
+                int nc = lcap ? 2 * lcap : 4096;
+                inj_cell *nb = (inj_cell *)realloc(lc, sizeof(inj_cell) * (size_t)nc);
+                if (!nb) {
+                    rc = -1;
+                    break;
+                }
+                lc = nb;
+                lcap = nc;
+            }
+            inj_cell *e = &lc[ln++];
+            e->q = LP(i, j, M);
+            e->call = call;
+            /* SW2 evaluates this exp twice, for eta and for the tracer:
+             * one call gives both values */
+            const double ex = exp(-(r * r) / (2.0 * (sigma * sigma)));
+            e->pf = amp * ex;
+            e->tv = (inject) * ex;
+        }
+        if (rc == 0) sort_row(lc + row0, ln - row0, slot, nslot, tmp);
+    }
+    *cells = lc;
+    *n = ln;
+    *cap = lcap;
+    free(slot);
+    free(tmp);
+    return rc;
+}
+
+/* the columns in descending cos(lonc - lon) order, as far as a walk may go.
+ * With 2H+1 < nx only the 2H+1 columns nearest to lon (cyclically) are
+ * evaluated and sorted; *ntrust = 2H+1-16 of them are the global top:
+ * every column outside the window is at least 8 column widths farther from
+ * lon than those, so its cosine is smaller by far more than any rounding.
+ * Otherwise all columns, all trusted. */
+static int sorted_cols(sw *SW, double lon, int H, col_cos *cols, int *ntrust) {
+    const int nx = SW->nx;
+    int m = 0;
+    if (2 * H + 1 >= nx) {
+        for (int i = 1; i <= nx; i++) {
+            cols[m].i = i;
+            cols[m++].c = cos(g_lonc(SW, i) - lon);
+        }
+        *ntrust = nx;
+    } else {
+        const double dl = (SW->lon1 - SW->lon0) / nx;
+        const int ic = (int)floor((lon - SW->lon0) / dl) + 1; /* the column (or its periodic copy) holding lon */
+        for (int d = -H; d <= H; d++) {
+            const int i = (((ic + d - 1) % nx) + nx) % nx + 1;
+            cols[m].i = i;
+            cols[m++].c = cos(g_lonc(SW, i) - lon);
+        }
+        *ntrust = m - 16;
+    }
+    qsort(cols, (size_t)m, sizeof(col_cos), col_cmp);
+    return m;
 }
 
 /* the cells get_Gaussian_field (perturbs.c:348-395) accepts for one centre,
@@ -859,14 +981,8 @@ static int col_cmp(const void *a, const void *b) {
  * comes within a relative 1e-9 of the radius. */
 static int gaussian_cells(sw *SW, double lon, double lat, double amp, double sigma, double radius, double inject, int call,
                           inj_cell **cells, int *n, int *cap, col_cos *cols) {
-    map *M = SW->M;
-    const int nx = SW->nx, ny = SW->ny;
+    const int ny = SW->ny;
     const double Rm = (SW->rE + SW->rP) / 2.0, lat0 = g_lat2pclat(SW, lat);
-    for (int i = 1; i <= nx; i++) {
-        cols[i - 1].i = i;
-        cols[i - 1].c = cos(g_lonc(SW, i) - lon);
-    }
-    qsort(cols, (size_t)nx, sizeof(col_cos), col_cmp);
     /* Rows whose (parametric) latitude lies farther than the radius from
      * the centre's hold no accepted cell: the central angle between two
      * points is at least their latitude difference, and the computed
@@ -893,44 +1009,66 @@ static int gaussian_cells(sw *SW, double lon, double lat, double amp, double sig
         }
         jhi = lo - 1;
     }
-    for (int j = jlo; j <= jhi; j++) {
-        const double lat1 = g_lat2pclat(SW, g_latc(SW, j));
-        const double a = sin(lat0) * sin(lat1), b = cos(lat0) * cos(lat1);
-        const double rbest = Rm * acos(a + b * cols[0].c);
-        if (!(rbest < radius * (1.0 + 1e-9))) continue;
-        const int row0 = *n;
-        int misses = 0;
-        for (int k = 0; k < nx && misses < 8; k++) {
-            const int i = cols[k].i;
-            /* great_circle(lon, lat0, lonc(i), lat1, Rm) term by term: its
-             * two latitude products are the row's a and b, and its
-             * cos(lonc(i) - lon) is cols[k].c, the same call on the same
-             * operands */
-            double r = Rm * acos(a + b * cols[k].c);
-            if (!(r < radius)) {
-                misses++;
-                continue;
-            }
-            misses = 0;
-            if (*n == *cap) {
-                int nc = *cap ? 2 * *cap : 4096;
-                inj_cell *nb = (inj_cell *)realloc(*cells, sizeof(inj_cell) * (size_t)nc);
-                if (!nb) return -1;
-                *cells = nb;
-                *cap = nc;
-            }
-            inj_cell *e = &(*cells)[(*n)++];
-            e->q = LP(i, j, M);
-            e->call = call;
-            /* SW2 evaluates this exp twice, for eta and for the tracer:
-             * one call gives both values */
-            const double ex = exp(-(r * r) / (2.0 * (sigma * sigma)));
-            e->pf = amp * ex;
-            e->tv = (inject) * ex;
-        }
-        sort_row(*cells + row0, *n - row0);
+    const int nrows = jhi - jlo + 1;
+    int nt = 1;
+#ifdef _OPENMP
+    /* rows are independent: a large band is split into contiguous chunks,
+     * one list per thread, joined in row order (the same cells in the same
+     * order as one thread) */
+    if (nrows >= 32) {
+        nt = omp_get_max_threads();
+        if (nt > 8) nt = 8;
+        if (nt > nrows / 16) nt = nrows / 16;
+        if (nt < 1) nt = 1;
     }
-    return 0;
+#endif
+    /* a window of columns first; wider (finally all of them) if a row's
+     * walk reaches past its trusted part */
+    const int n0 = *n;
+    for (int H = 128;; H *= 2) {
+        int ntrust;
+        sorted_cols(SW, lon, H, cols, &ntrust);
+        int r;
+        if (nt == 1) {
+            r = band_cells(SW, jlo, jhi, lat0, Rm, amp, sigma, radius, inject, call, cells, n, cap, cols, ntrust);
+        } else {
+            r = 0;
+            inj_cell *tc[8] = {NULL};
+            int tn[8] = {0}, tcap[8] = {0}, terr[8] = {0};
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt)
+            {
+                const int id = omp_get_thread_num();
+                const int r0 = jlo + (int)((long)nrows * id / nt), r1 = jlo + (int)((long)nrows * (id + 1) / nt) - 1;
+                terr[id] = band_cells(SW, r0, r1, lat0, Rm, amp, sigma, radius, inject, call, &tc[id], &tn[id], &tcap[id],
+                                      cols, ntrust);
+            }
+#endif
+            int total = *n;
+            for (int t = 0; t < nt; t++) {
+                if (terr[t] < 0) r = -1;
+                else if (terr[t] > 0 && r == 0) r = 1;
+                total += tn[t];
+            }
+            if (r == 0 && total > *cap) {
+                inj_cell *nb = (inj_cell *)realloc(*cells, sizeof(inj_cell) * (size_t)total);
+                if (nb) {
+                    *cells = nb;
+                    *cap = total;
+                } else
+                    r = -1;
+            }
+            for (int t = 0; t < nt; t++) {
+                if (r == 0 && tn[t]) {
+                    memcpy(*cells + *n, tc[t], sizeof(inj_cell) * (size_t)tn[t]);
+                    *n += tn[t];
+                }
+                free(tc[t]);
+            }
+        }
+        if (r <= 0) return r;
+        *n = n0; /* the window was too narrow: again, wider */
+    }
 }
 
 /* merge the cell lists of the calls (each in field order, calls in order)

@@ -76,6 +76,18 @@ def test_sparse_injection_equals_full_grid(name, t):
             assert np.isnan(inc.ravel()[0])
 
 
+def test_sparse_injection_split_over_threads():
+    """A band of more than 32 rows is split over the host's OpenMP threads;
+    the joined lists are the full-grid cells in SW2's order all the same."""
+    G = [[0.0, 5000.0, 100.0, 359.5, 20.0, 1, 0.5, 2e11, 1, 5.0, 14.0, 1.0]]
+    K, K2, out = both(G, 600.0, nx=1024, ny=512)
+    assert K == K2 == 1
+    (inc, trv, mask), (idx, sv, st, has_inc) = out[0]
+    assert len(idx) > 3000
+    assert np.array_equal(np.flatnonzero(mask), idx)
+    assert np.array_equal(inc[idx], sv) and np.array_equal(trv[idx], st)
+
+
 def test_zero_volume_moment_leaves_eta_alone():
     # Q(t) at t0 is zero: SW2 marks the tracer but adds nothing to eta
     K, K2, out = both([[100.0, 5000.0, 100.0, 30.0, 10.0, 1, 0.0, 2e11, 1, 4.0, 8.0, 1.0]], 100.0)
