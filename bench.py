@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--ensemble-members", type=int, default=64, help="members of the batched ensemble (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=3000)
     ap.add_argument("--served-requests", type=int, default=40, help="requests of the served stream (0: skip)")
+    ap.add_argument("--storm-steps", type=int, default=500,
+                    help="steps of the 4096x2048 run with a moving storm, fused and phased (0: skip)")
     return ap.parse_args()
 
 
@@ -167,6 +169,40 @@ def bench_mixed(steps):
     for w in worlds.values():
         w.close()
     return out
+
+
+def bench_storm(steps):
+    """The FP64 workload's case with a moving storm that injects tracer
+    (perturbs.c's Gaussians): the fused step (injections after the momentum
+    kernel, tracer advected every step), then a shorter phased run of the
+    same case for comparison."""
+    import numpy as np
+    from sw2_amd import Session, cases
+    G = np.array([[0.0, 1e9, 100.0, 30.0, 15.0, 0, 20.0, 3e11, 1, 3.0, 6.0, 1.0]])
+    res = {}
+    for label, form, n in (("fused", 0, steps), ("phased", 0x100, max(20, steps // 5))):
+        progress(f"storm case: {label} step, {n} steps")
+        p = cases.config_params("c1", steps=n + 10, Gaussians=G)
+        w, _ = cases.make_world(p, cases.single_bump(2024))
+        s = Session(w, form)
+        s.begin(0, p.t0)
+        s.step(5)
+        s.timed_steps(0)
+        t0 = time.perf_counter()
+        dev_ms = s.timed_steps(n)
+        wall = time.perf_counter() - t0
+        kt = s.kernel_times(20) if s.fused else None
+        res[label] = {"ms_per_step": wall * 1e3 / n, "device_ms_per_step": dev_ms / n, "fused": s.fused,
+                      "failed_step": s.failed_step()[0],
+                      "kernel_us": {"kf_eta": kt[0] / 20 * 1e3, "kf_uv": kt[1] / 20 * 1e3} if kt else None}
+        s.close()
+        w.close()
+        cells = p.nx * p.ny
+    f, ph = res["fused"], res["phased"]
+    return {"workload": f"{p.nx}x{p.ny} FP64 jet + moving 6-degree storm injecting tracer, {steps} steps",
+            "value": cells / (f["ms_per_step"] / 1e3), "unit": "cell-updates/s", "dtype": "f64",
+            "ms_per_step": f["ms_per_step"], "kernel_us": f["kernel_us"], "failed_step": f["failed_step"],
+            "phased_ms_per_step": ph["ms_per_step"], "speedup_vs_phased": ph["ms_per_step"] / f["ms_per_step"]}
 
 
 def fp64_step_ms(nx, ny, steps=40):
@@ -313,6 +349,8 @@ def main():
         extras["ensemble"] = bench_ensemble(a.ensemble_members, a.ensemble_steps)
     if a.served_requests > 0:
         extras["served"] = bench_served(a.served_requests)
+    if a.storm_steps > 0 and wl == "c1":
+        extras["storm"] = bench_storm(a.storm_steps)
     out = {
         "metric": "cell-updates/s", "value": value, "unit": "cell-updates/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall * 1e3 / a.steps,

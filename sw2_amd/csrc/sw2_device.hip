@@ -545,6 +545,31 @@ __global__ void k_zero_pert(Flags *fl, int step, int S, double *f) {
 #include "sw2_mixed.inc"
 
 namespace {
+// storm increments (eta-only storms) on the mixed form's FP32 height anomaly
+// eta' (eta = hbar(j) + eta'), rounded once from FP64, with the periodic
+// images the height kernel writes; the marks as k_inject sets them
+__global__ void k_inject_mixed(Grid g, Flags *fl, int step, int n, const int *idx, const double *inc, int has_inc, float *eta,
+                               double *pert) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((fl->fail_step && (fl->fail_step != step || fl->fail_kind == 1)) || k >= n) return;
+    const int q = idx[k];
+    pert[q] = 1.0;
+    if (!has_inc) return;
+    const int c = q % g.sx - 1, j = q / g.sx - 1;
+    mput_periodic<float, true>(g, eta, c, j, (float)((double)eta[q] + inc[k]));
+}
+
+// check_stats_scaf on the mixed form's eta = hbar(j) + eta' (the height
+// check the momentum kernel leaves for after the storms)
+__global__ __launch_bounds__(BX *BY) void k_check_mixed_eta(Grid g, Flags *fl, const float *eta, const double *hbar, int n) {
+    const int i = 1 + blockIdx.x * BX + threadIdx.x;
+    const int j = 1 + blockIdx.y * BY + threadIdx.y;
+    if (fl->fail_step && fl->fail_step != n) return;
+    if (i > g.nx || j > g.ny) return;
+    const double x = hbar[j + 1] + (double)eta[IX(g, i, j)];
+    if ((x < -1e10) || (x > 1e10) || (i == 1 && j == 1 && x != x)) flag_bad(fl, n, 3);
+}
+
 // device reductions of SW2's energy diagnostics (energy.c:25-55) and of the
 // mass sum H*Dxc*Dyc over the interior, in FP64, for either form: per-block
 // partial sums, then one block adding them in a fixed order
@@ -682,6 +707,10 @@ static int mixed_setup(sw2_session *s, sw *SW) {
     for (auto f : fs) rc |= falloc(f, S);
     if (!rc && hipMalloc((void **)&s->d_cflp, sizeof(unsigned long long) * 2 * NPART) != hipSuccess) rc = -1;
     if (!rc && hipMemset(s->d_cflp, 0, sizeof(unsigned long long) * 2 * NPART) != hipSuccess) rc = -1;
+    if (!rc && SW->num_Gaussians) { /* the storms' marks, as SW2's perturbs field */
+        rc = dalloc(&s->pert, S);
+        if (!rc && hipMemcpy(s->pert, SW->perturbs, sizeof(double) * S, hipMemcpyHostToDevice) != hipSuccess) rc = -1;
+    }
     for (int k = 0; k < 3; k++) {
         rc |= falloc(&s->m_De[k], S);
         rc |= falloc(&s->m_Du[k], S);
@@ -784,9 +813,9 @@ extern "C" int sw2_session_create(sw *SW, int form, sw2_session **out) {
     s->fused = !force_phased && tables_ok && wind_rows_ok && windy_zero && SW->num_vortices == 0 && SW->nu6 == 0 &&
                SW->nx >= 2 * FTX && SW->ny >= 4 && !(SW->nu2 != 0 && SW->nu4 == 0);
     s->hvmode = s->hv ? 2 : 0;
-    if (form == SW2_FORM_MIXED && !(s->fused && !s->tracer_active && SW->num_Gaussians == 0)) {
-        sw2_set_error("the mixed-precision form needs a case the fused step takes: no in-loop storms or vortices, "
-                      "no del-6, a zero tracer, a j-only wind, nx >= %d, ny >= 4", 2 * FTX);
+    if (form == SW2_FORM_MIXED && !(s->fused && !s->tracer_active)) {
+        sw2_set_error("the mixed-precision form needs a case the fused step takes: no vortices, no del-6, a zero tracer "
+                      "that no storm injects into, a j-only wind, nx >= %d, ny >= 4", 2 * FTX);
         free(tab);
         free(s);
         return -1;
@@ -926,7 +955,10 @@ static void lape(sw2_session *s, const double *ein, double *eo) {
     k_lape_fix<<<(s->g.nx + 255) / 256, 256, 0, s->st>>>(s->g, s->fl, eo);
 }
 
-static int host_perturbations(sw2_session *s, int n, double t, int *eta_changed) {
+// storms (and, FP64 only, vortices) of step n at time t: on stream st, into
+// the session's fields (meta: the FP32 height anomaly that receives them in
+// the mixed form)
+static int host_perturbations_on(sw2_session *s, int n, double t, int *eta_changed, hipStream_t st, float *meta) {
     sw *SW = s->SW;
     *eta_changed = 0;
     if (SW->num_Gaussians == 0 && SW->num_vortices == 0) return 0;
@@ -967,7 +999,7 @@ static int host_perturbations(sw2_session *s, int n, double t, int *eta_changed)
             memcpy(h_trv, x.trv, sizeof(double) * (size_t)x.n);
             off += sizeof(double) * (size_t)x.n;
             if (x.n > s->d_cap[k]) {
-                HIPCHECK(hipStreamSynchronize(s->st));
+                HIPCHECK(hipStreamSynchronize(st));
                 if (s->d_idx[k]) HIPCHECK(hipFree(s->d_idx[k]));
                 if (s->d_inc[k]) HIPCHECK(hipFree(s->d_inc[k]));
                 if (s->d_trv[k]) HIPCHECK(hipFree(s->d_trv[k]));
@@ -980,15 +1012,19 @@ static int host_perturbations(sw2_session *s, int n, double t, int *eta_changed)
                 HIPCHECK(hipMalloc((void **)&s->d_trv[k], sizeof(double) * (size_t)cap));
                 s->d_cap[k] = cap;
             }
-            HIPCHECK(hipMemcpyAsync(s->d_idx[k], h_idx, sizeof(int) * (size_t)x.n, hipMemcpyHostToDevice, s->st));
-            HIPCHECK(hipMemcpyAsync(s->d_inc[k], h_inc, sizeof(double) * (size_t)x.n, hipMemcpyHostToDevice, s->st));
-            HIPCHECK(hipMemcpyAsync(s->d_trv[k], h_trv, sizeof(double) * (size_t)x.n, hipMemcpyHostToDevice, s->st));
-            k_inject<<<(x.n + 255) / 256, 256, 0, s->st>>>(s->fl, n, x.n, s->d_idx[k], s->d_inc[k], s->d_trv[k], x.has_inc,
-                                                          s->eta, s->tr, s->pert);
+            HIPCHECK(hipMemcpyAsync(s->d_idx[k], h_idx, sizeof(int) * (size_t)x.n, hipMemcpyHostToDevice, st));
+            HIPCHECK(hipMemcpyAsync(s->d_inc[k], h_inc, sizeof(double) * (size_t)x.n, hipMemcpyHostToDevice, st));
+            HIPCHECK(hipMemcpyAsync(s->d_trv[k], h_trv, sizeof(double) * (size_t)x.n, hipMemcpyHostToDevice, st));
+            if (s->form == SW2_FORM_MIXED)
+                k_inject_mixed<<<(x.n + 255) / 256, 256, 0, st>>>(s->g, s->fl, n, x.n, s->d_idx[k], s->d_inc[k], x.has_inc,
+                                                                    meta, s->pert);
+            else
+                k_inject<<<(x.n + 255) / 256, 256, 0, st>>>(s->fl, n, x.n, s->d_idx[k], s->d_inc[k], s->d_trv[k], x.has_inc,
+                                                              s->eta, s->tr, s->pert);
             s->perturbs_dirty = 1;
             if (x.has_inc) *eta_changed = 1;
         }
-        HIPCHECK(hipEventRecord(s->stage_ev[b], s->st));
+        HIPCHECK(hipEventRecord(s->stage_ev[b], st));
     }
     if (SW->num_vortices) {
         if (!s->h_dv[0][0]) {
@@ -1000,20 +1036,24 @@ static int host_perturbations(sw2_session *s, int n, double t, int *eta_changed)
         if (V < 0) return -1;
         dim3 G = cells_grid(s->g), B = cells_block();
         for (int k = 0; k < V; k++) {
-            HIPCHECK(hipMemcpyAsync(s->pinc, s->h_dv[0][k], sizeof(double) * S, hipMemcpyHostToDevice, s->st));
-            HIPCHECK(hipMemcpyAsync(s->pdu, s->h_dv[1][k], sizeof(double) * S, hipMemcpyHostToDevice, s->st));
-            HIPCHECK(hipMemcpyAsync(s->pdv, s->h_dv[2][k], sizeof(double) * S, hipMemcpyHostToDevice, s->st));
-            k_add<<<G, B, 0, s->st>>>(s->g, s->fl, s->eta, s->pinc);
-            k_add<<<G, B, 0, s->st>>>(s->g, s->fl, s->u, s->pdu);
-            k_add<<<G, B, 0, s->st>>>(s->g, s->fl, s->v, s->pdv);
+            HIPCHECK(hipMemcpyAsync(s->pinc, s->h_dv[0][k], sizeof(double) * S, hipMemcpyHostToDevice, st));
+            HIPCHECK(hipMemcpyAsync(s->pdu, s->h_dv[1][k], sizeof(double) * S, hipMemcpyHostToDevice, st));
+            HIPCHECK(hipMemcpyAsync(s->pdv, s->h_dv[2][k], sizeof(double) * S, hipMemcpyHostToDevice, st));
+            k_add<<<G, B, 0, st>>>(s->g, s->fl, s->eta, s->pinc);
+            k_add<<<G, B, 0, st>>>(s->g, s->fl, s->u, s->pdu);
+            k_add<<<G, B, 0, st>>>(s->g, s->fl, s->v, s->pdv);
             launch_halo(s, s->eta);
             launch_halo(s, s->u);
             launch_halo(s, s->v);
             *eta_changed = 1;
-            HIPCHECK(hipStreamSynchronize(s->st));  // host scratch is reused
+            HIPCHECK(hipStreamSynchronize(st));  // host scratch is reused
         }
     }
     return 0;
+}
+
+static int host_perturbations(sw2_session *s, int n, double t, int *eta_changed) {
+    return host_perturbations_on(s, n, t, eta_changed, s->st, s->m_eta);
 }
 
 template <int SCH, int HV>
@@ -1095,7 +1135,7 @@ static void launch_mixed(sw2_session *s, int n, int k, int slot) {
     km_eta<SCH, HV><<<G, NTE, 0, s->st>>>(s->g, s->fl, ea);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 1], s->st);
     MUVArgs ua{s->m_eta2, s->m_u, s->m_v, s->m_Du[1], s->m_Du[2], s->m_Dv[1], s->m_Dv[2], s->m_u2, s->m_v2, s->m_Du[0],
-               s->m_Dv[0], s->d_mt, s->d_cflp, n, k, s->Dt, s->sigma, s->c2, s->c4};
+               s->m_Dv[0], s->d_mt, s->d_cflp, n, k, s->Dt, s->sigma, s->c2, s->c4, s->SW->num_Gaussians > 0};
     km_uv<SCH, HV><<<G, NTU, 0, s->st>>>(s->g, s->fl, ua);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 2], s->st);
     km_cfl_parts<<<1, 64, 0, s->st>>>(s->d_cflp, s->fl, slot ^ 1);
@@ -1122,8 +1162,19 @@ static int step_mixed(sw2_session *s) {
     frot(s->m_De);
     frot(s->m_Du);
     frot(s->m_Dv);
+    const double t = s->t + s->Dt;
+    if (s->SW->num_Gaussians) {
+        // eta-only storms after u and v, then the height check (core.c:280-312)
+        if (s->perturbs_dirty) {
+            k_zero_pert<<<(s->S + 255) / 256, 256, 0, s->st>>>(s->fl, n, s->S, s->pert);
+            s->perturbs_dirty = 0;
+        }
+        int eta_changed = 0;
+        if (host_perturbations(s, n, t, &eta_changed)) return -1;
+        k_check_mixed_eta<<<cells_grid(s->g), cells_block(), 0, s->st>>>(s->g, s->fl, s->m_eta, s->d_mt + MT_HBAR * s->g.R, n);
+    }
     s->n = n;
-    s->t = s->t + s->Dt;
+    s->t = t;
     return 0;
 }
 
@@ -1299,6 +1350,7 @@ extern "C" int sw2_session_download(sw2_session *s, sw *SW) {
                 }
         }
         free(h);
+        if (!rc && s->pert && hipMemcpy(SW->perturbs, s->pert, sizeof(double) * S, hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
         if (rc) sw2_set_error("download failed");
         return rc;
     }
@@ -1412,6 +1464,8 @@ struct sw2_batch {
     hipGraph_t graph;
     hipGraphExec_t gexec;
     int gsteps, blocks_eta, blocks_uv, table_dirty;
+    int storms;   // members with in-loop storms: steps launched one by one, storms after each
+    int *paused;  // host copy of the members' paused flags
 };
 
 extern "C" size_t sw2_batch_footprint(const sw *SW, int form) {
@@ -1419,6 +1473,7 @@ extern "C" size_t sw2_batch_footprint(const sw *SW, int form) {
     size_t f = (size_t)(SW2_NT + 2) * R * sizeof(double) + 3 * R * sizeof(double) + (3 * 1024 + 3) * sizeof(double) + sizeof(Flags);
     if (form == SW2_FORM_MIXED) f += 15 * S * sizeof(float);
     else f += 24 * S * sizeof(double);
+    if (SW->num_Gaussians) f += S * sizeof(double); /* the storms' marks */
     return f;
 }
 
@@ -1446,6 +1501,7 @@ extern "C" int sw2_batch_create(int form, int capacity, double hbm_budget_bytes,
     b->sess = (sw2_session **)calloc(capacity, sizeof(sw2_session *));
     b->foot = (size_t *)calloc(capacity, sizeof(size_t));
     b->h_mem = (BMember *)calloc(capacity, sizeof(BMember));
+    b->paused = (int *)calloc(capacity, sizeof(int));
     b->gsteps = 8;
     // default persistent grids: as many workgroups as stay resident at once
     // (occupancy of the minmod, del-4 instantiations), so that the static
@@ -1549,6 +1605,9 @@ extern "C" int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id) {
     M.courant = s->courant;
     M.rest_x = s->rest_x;
     M.rest_y = s->rest_y;
+    M.late = SW->num_Gaussians > 0;
+    if (M.late) b->storms++;
+    b->paused[slot] = 0;
     b->sess[slot] = s;
     b->foot[slot] = foot;
     b->used += foot;
@@ -1637,9 +1696,43 @@ extern "C" int sw2_batch_get_launch(sw2_batch *b, int *blocks_eta, int *blocks_u
 
 // advance every active member by nsteps (members stop at their own last
 // step): whole graphs of gsteps steps, the remainder launched directly
+// after a batch step: the storms of every member that took it, into the
+// buffer holding its new height, and its height check after them (as a
+// single mixed session orders them).  The member's session counts its
+// steps and time as a single session does (time by repeated addition of
+// Dt), so its storms are those of its own run.
+static int batch_storms(sw2_batch *b) {
+    for (int m = 0; m < b->cap; m++) {
+        sw2_session *s = b->sess[m];
+        const BMember &M = b->h_mem[m];
+        if (!s || !M.late || b->paused[m] || s->n >= M.nlast) continue;
+        const int n = s->n + 1;
+        const double t = s->t + s->Dt;
+        float *eta_new = M.eta[((n - M.nstart - 1) & 1) ^ 1];
+        if (s->perturbs_dirty) {
+            k_zero_pert<<<(s->S + 255) / 256, 256, 0, b->st>>>(s->fl, n, s->S, s->pert);
+            s->perturbs_dirty = 0;
+        }
+        int eta_changed = 0;
+        if (host_perturbations_on(s, n, t, &eta_changed, b->st, eta_new)) return -1;
+        k_check_mixed_eta<<<cells_grid(s->g), cells_block(), 0, b->st>>>(s->g, s->fl, eta_new, s->d_mt + MT_HBAR * s->g.R, n);
+        s->n = n;
+        s->t = t;
+    }
+    return 0;
+}
+
 extern "C" int sw2_batch_step(sw2_batch *b, int nsteps) {
     if (b->nmem == 0 || nsteps <= 0) return 0;
     if (batch_upload(b)) return -1;
+    if (b->storms) {
+        for (int q = 0; q < nsteps; q++) {
+            batch_launch_step(b);
+            if (batch_storms(b)) return -1;
+        }
+        HIPCHECK(hipGetLastError());
+        return 0;
+    }
     if (batch_capture(b)) return -1;
     int q = 0;
     for (; q + b->gsteps <= nsteps; q += b->gsteps) HIPCHECK(hipGraphLaunch(b->gexec, b->st));
@@ -1678,6 +1771,7 @@ extern "C" int sw2_batch_pause(sw2_batch *b, int id, int paused) {
     HIPCHECK(hipMemcpyAsync(reinterpret_cast<char *>(b->d_st + id) + offsetof(BState, paused), &v, sizeof v,
                             hipMemcpyHostToDevice, b->st));
     HIPCHECK(hipStreamSynchronize(b->st));
+    b->paused[id] = v;
     return 0;
 }
 
@@ -1709,8 +1803,10 @@ extern "C" int sw2_batch_download(sw2_batch *b, int id, sw *SW) {
     s->m_u2 = M.u[p ^ 1];
     s->m_v = M.v[p];
     s->m_v2 = M.v[p ^ 1];
-    s->n = steps;
-    s->t = SW->t0 + steps * s->Dt;
+    if (!M.late) { /* a storm member's session already counts its steps and time */
+        s->n = steps;
+        s->t = SW->t0 + steps * s->Dt;
+    }
     return sw2_session_download(s, SW);
 }
 
@@ -1734,6 +1830,7 @@ extern "C" int sw2_batch_remove(sw2_batch *b, int id) {
     HIPCHECK(hipStreamSynchronize(b->st));
     BState z{0, 1, 0};
     HIPCHECK(hipMemcpy(b->d_st + id, &z, sizeof z, hipMemcpyHostToDevice));
+    if (b->h_mem[id].late) b->storms--;
     memset(&b->h_mem[id], 0, sizeof(BMember));
     batch_retile(b);
     // the member's device arrays are released only after the table that
@@ -1766,6 +1863,7 @@ extern "C" void sw2_batch_destroy(sw2_batch *b) {
     free(b->sess);
     free(b->foot);
     free(b->h_mem);
+    free(b->paused);
     free(b);
 }
 

@@ -7,7 +7,7 @@ form's tolerance of the FP64 oracle (tests/test_gpu_mixed.py)."""
 import numpy as np
 import pytest
 
-from sw2_amd import Batch, FORM_MIXED, Session, cases
+from sw2_amd import Batch, FORM_MIXED, Session, SW2Error, cases
 
 pytestmark = pytest.mark.gpu
 
@@ -114,4 +114,41 @@ def test_admission_by_hbm_footprint():
     assert b.count() == (2, 2 * foot)
     b.remove(ids[0])
     assert b.add(cases.make_world(p, [])[0], 10) is not None
+    b.close()
+
+
+def test_batch_members_with_storms_equal_solo_runs():
+    """Members with height-only storms (one crossing the seam and running
+    past the member's last step, one a sink whose window closes mid-run) of
+    different sizes, beside a member without storms.  One storm member is
+    held for 20 steps and released.  Each member's result, storm marks
+    included, equals the same world run alone in a mixed session."""
+    G1 = np.array([[0.0, 1e5, 200.0, 358.0, 20.0, 1, 0.4, 3e12, 1, 3.0, 8.0, 0.0]])
+    G2 = np.array([[100.0, 3000.0, 100.0, 150.0, -30.0, 0, -40.0, -1e12, 1, 2.5, 6.0, 0.0]])
+    specs = [(256, 128, G1, 120), (512, 256, G2, 90), (384, 192, None, 100)]
+    b = Batch(capacity=4)
+    members = []
+    for nx, ny, G, n in specs:
+        kw = {} if G is None else {"Gaussians": G}
+        p = cases.scaled_params(nx, ny, steps=n, lat=70.0, **kw)
+        bumps = cases.single_bump(nx)
+        w, _ = cases.make_world(p, bumps)
+        members.append((b.add(w, n), p, bumps, n))
+    b.step(30)
+    b.sync()
+    b.pause(members[0][0])
+    b.step(20)
+    b.sync()
+    assert b.member(members[0][0])["steps"] == 30
+    b.pause(members[0][0], False)
+    b.step(200)  # members stop at their own last step
+    b.sync()
+    for mid, p, bumps, n in members:
+        st = b.member(mid)
+        assert st["done"] and st["steps"] == n and st["failed_step"] == 0, st
+        w = b.download(mid)
+        ref = solo(p, bumps, n)
+        same(w, ref, f"storm member {mid}")
+        assert np.array_equal(w.field("perturbs"), ref.field("perturbs"))
+    assert b.download(members[0][0]).field("perturbs").max() == 1.0
     b.close()

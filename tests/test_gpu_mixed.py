@@ -125,10 +125,49 @@ def test_mixed_refuses_cases_the_fused_step_does_not_take():
     w, _ = cases.make_world(p, cases.single_bump(1))
     with pytest.raises(SW2Error, match="mixed-precision form"):
         Session(w, FORM_MIXED)
-    # storms run on the fused FP64 step, not in the mixed form
+    # storms that inject tracer run on the fused FP64 step only (the mixed
+    # form carries no tracer)
     G = np.array([[0.0, 3000.0, 100.0, 100.0, 10.0, 1, 1.0, 2e11, 1, 4.0, 10.0, 1.0]])
     p = cases.scaled_params(128, 32, steps=5, lat=60.0, Gaussians=G)
     w, _ = cases.make_world(p, cases.single_bump(1))
     assert Session(w).fused
     with pytest.raises(SW2Error, match="mixed-precision form"):
         Session(w, FORM_MIXED)
+
+
+def test_mixed_eta_only_storms_against_fp64():
+    """Storms that leave the tracer alone (inject 0, as in the reference's
+    JupiterEnergyTest and SaturnSmoothInjection cases) in the mixed form:
+    the same storm cells marked as the FP64 run's, and eta, u, v within the
+    mixed form's tolerance of the FP64 GPU run, which the parity tests hold
+    bit-equal to the oracle.  One storm crosses the periodic seam, one is a
+    sink."""
+    # the first storm is still injecting at the last step; the second stops
+    # a third of the way in
+    G = np.array([[0.0, 1e5, 200.0, 358.0, 20.0, 1, 0.4, 3e12, 1, 3.0, 8.0, 0.0],
+                  [300.0, 4000.0, 100.0, 150.0, -30.0, 0, -40.0, -1e12, 1, 2.5, 6.0, 0.0]])
+    p = cases.scaled_params(512, 256, steps=300, lat=75.0, Gaussians=G)
+    bumps = cases.single_bump(31)
+    runs = {}
+    for form in (FORM_FP64, FORM_MIXED):
+        w, _ = cases.make_world(p, bumps)
+        s = Session(w, form)
+        assert s.form == form and s.fused
+        s.begin(0, p.t0)
+        s.step(300)
+        s.download()
+        runs[form] = (w, s.failed_step()[0])
+    (w64, f64), (w32, f32) = runs[FORM_FP64], runs[FORM_MIXED]
+    assert f64 == f32 == 0
+    assert np.array_equal(w32.field("perturbs"), w64.field("perturbs"))
+    assert w64.field("perturbs").max() == 1.0
+    e = report("mixed eta-only storms 300 steps", w32, w64.interior)
+    for f, (l2, mx) in e.items():
+        assert l2 <= TOL_L2, (f, l2)
+    # the storms moved the height: the comparison is not of two quiet runs
+    w0, _ = cases.make_world(cases.scaled_params(512, 256, steps=300, lat=75.0), bumps)
+    s0 = Session(w0, FORM_FP64)
+    s0.begin(0, p.t0)
+    s0.step(300)
+    s0.download()
+    assert errors(w64.interior("eta"), w0.interior("eta"))[0] > 100 * TOL_L2
