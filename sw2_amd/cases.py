@@ -116,10 +116,11 @@ def storm_bumps(rng):
              float(rng.uniform(1.0, 4.0))) for _ in range(k)]
 
 
-def member_params(nx, ny, steps):
+def member_params(nx, ny, steps, **kw):
     """The 360x180 workload's physics on an nx x ny grid (lat -80..80): Dt
-    and the del-4 coefficient scaled with the grid spacing."""
-    return scaled_params(nx, ny, steps=steps, lat=80.0)
+    and the del-4 coefficient scaled with the grid spacing; keyword
+    arguments as jet_params takes them (an in-loop storm table, say)."""
+    return scaled_params(nx, ny, steps=steps, lat=80.0, **kw)
 
 
 def ensemble(n=64, steps=3000, seed=2026):

@@ -1,6 +1,7 @@
 """Serving SW2 runs: a continuous-batching request loop over sw2_batch_*.
 
-Requests (arrival time, grid, steps, storm bumps) arrive on their own
+Requests (arrival time, grid, steps, initial storm bumps, and optionally a
+table of in-loop storms in SW2's Gaussian-table columns) arrive on their own
 schedule.  The loop admits a waiting request as soon as a member slot and
 the HBM budget allow (sw2_batch_add checks the member's footprint), keeps
 every admitted run's state on the device, advances all of them together in
@@ -33,8 +34,10 @@ from . import Batch, cases
 
 
 def _build(req):
-    _, nx, ny, steps, bumps = req
-    w, _ = cases.make_world(cases.member_params(nx, ny, steps), bumps)
+    _, nx, ny, steps, bumps = req[:5]
+    storms = req[5] if len(req) > 5 else None
+    kw = {} if storms is None else {"Gaussians": np.asarray(storms, dtype=float)}
+    w, _ = cases.make_world(cases.member_params(nx, ny, steps, **kw), bumps)
     return w
 
 
@@ -63,7 +66,7 @@ class Server:
                 wq[2] = self.pool.submit(_build, wq[1])
 
     def run(self, requests, on_result=None, max_wall=None, log=None):
-        """Serve `requests` = [(t_arrival_s, nx, ny, steps, bumps), ...] in
+        """Serve `requests` = [(t_arrival_s, nx, ny, steps, bumps[, storms]), ...] in
         real time; returns the statistics (and calls on_result(k, batch,
         member_id) before each run is retired, for callers that keep the
         results).  Waiting requests are admitted in arrival order; the
@@ -91,7 +94,8 @@ class Server:
             # already built are taken, with none the head is waited for
             while waiting:
                 self._prefetch(waiting)
-                k, (ta, nx, ny, steps, bumps), fut = waiting[0]
+                k, req, fut = waiting[0]
+                ta, nx, ny, steps = req[:4]
                 if live and not fut.done():
                     break
                 w = fut.result()

@@ -82,3 +82,38 @@ def test_srpt_holds_long_runs_and_results_stay_solo():
         ref = solo(nx, ny, steps, bumps)
         for f in ("eta", "u", "v"):
             assert np.array_equal(got[k][f], ref[f]), (k, f)
+
+
+def test_served_requests_with_in_loop_storms_equal_solo_runs():
+    """Requests carrying in-loop storms (height only) beside requests
+    without: the storm members step without the graph and inject after
+    every step; every result, storm marks included, equals its solo run."""
+    storm = np.array([[0.0, 1e5, 150.0, 359.0, 15.0, 1, 0.5, 2e12, 1, 3.0, 8.0, 0.0]])
+    sink = np.array([[50.0, 2500.0, 100.0, 120.0, -25.0, 0, -30.0, -8e11, 1, 2.5, 6.0, 0.0]])
+    rng = np.random.default_rng(13)
+    reqs = [(0.0, 256, 128, 80, cases.storm_bumps(rng), storm), (0.02, 512, 256, 60, cases.storm_bumps(rng)),
+            (0.04, 384, 192, 70, cases.storm_bumps(rng), sink), (0.06, 256, 128, 50, cases.storm_bumps(rng))]
+    got = {}
+
+    def keep(k, batch, mid):
+        w = batch.download(mid)
+        got[k] = {f: w.field(f).copy() for f in ("eta", "u", "v", "perturbs")}
+
+    srv = Server(capacity=4, tick_steps=10, policy="srpt")
+    st = srv.run(reqs, on_result=keep)
+    srv.close()
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    for k, req in enumerate(reqs):
+        _, nx, ny, steps, bumps = req[:5]
+        kw = {} if len(req) < 6 else {"Gaussians": req[5]}
+        w, _ = cases.make_world(cases.member_params(nx, ny, steps, **kw), bumps)
+        s = Session(w, FORM_MIXED)
+        s.begin(0, w.params.t0)
+        s.step(steps)
+        assert s.failed_step()[0] == 0
+        s.download()
+        for f in ("eta", "u", "v", "perturbs"):
+            assert np.array_equal(got[k][f], w.field(f)), (k, f)
+        s.close()
+        w.close()
+    assert got[0]["perturbs"].max() == 1.0
