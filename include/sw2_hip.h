@@ -306,10 +306,11 @@ int sw2_batch_create(int form, int capacity, double hbm_budget_bytes, sw2_batch 
  * Returns 0 and the member id, 1 when there is no room (slots or HBM
  * budget; nothing changed), or a negative error. */
 int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id);
-/* Launch shape: persistent workgroups of the height and momentum kernels and
- * steps per captured graph (the autotuner's knobs).  A new batch starts
- * with the workgroups that stay resident at once on the device and 8 steps
- * per graph. */
+/* Launch shape: workgroups of the height and momentum kernels and steps per
+ * captured graph (the autotuner's knobs).  0 workgroups: one per tile of
+ * the batch's members (the graph is captured again when members join or
+ * leave); n > 0: n persistent workgroups striding over the tiles.  A new
+ * batch starts with one workgroup per tile and 8 steps per graph. */
 int sw2_batch_set_launch(sw2_batch *b, int blocks_eta, int blocks_uv, int steps_per_graph);
 int sw2_batch_get_launch(sw2_batch *b, int *blocks_eta, int *blocks_uv, int *steps_per_graph);
 /* Queue nsteps steps of every active member (each stops at its own last

@@ -53,7 +53,12 @@ class Server:
         self.pool = ThreadPoolExecutor(max_workers=builders)
         self.prefetch = prefetch
         self.policy = policy
-        self.fill_tiles = fill * max(self.batch.launch()[:2])
+        # tiles that keep the device busy: `fill` per workgroup that can be
+        # resident at once (four of each batch kernel per compute unit), or
+        # per persistent workgroup when the launch shape sets them
+        from . import tune
+        resident = max(self.batch.launch()[:2]) or 4 * tune.device_cus()
+        self.fill_tiles = fill * resident
 
     def close(self):
         self.pool.shutdown(wait=True)

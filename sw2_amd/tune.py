@@ -63,7 +63,7 @@ def tune(sizes, steps=16, cache=CACHE, verbose=False):
         worlds.append(w)
         b.add(w, 1 << 30)
     t_start = time.perf_counter()
-    default = b.launch()  # the resident workgroups of each kernel, 8 steps per graph
+    default = b.launch()  # one workgroup per tile, 8 steps per graph
     best = default
     best_t = _time_steps(b, best, steps)
     tried = {best: best_t}
