@@ -155,6 +155,25 @@ def test_fused_storms_with_a_moving_tracer(sch):
     assert_same(w, o, f"fused storms sch={sch}")
 
 
+@pytest.mark.parametrize("nx,ny,nu2", [(200, 33, 0.0), (257, 64, 3e8)])
+def test_fused_storms_on_ragged_grids(nx, ny, nu2):
+    """Storms on grids that do not divide into tiles (the seam tile and the
+    last tile of a row band partly outside the domain), one crossing the
+    periodic seam at high latitude, with del-2 as well as del-4.  The
+    257x64 run with del-2 goes unstable near the pole and SW2's guard stops
+    it (step 37 in the oracle): both runs must stop on the same step with
+    the same frozen state."""
+    G = np.array([[0.0, 4000.0, 100.0, 358.0, 62.0, 1, 0.6, 2e11, 1, 5.0, 12.0, 1.0],
+                  [200.0, 3000.0, 80.0, 90.0, -20.0, 0, 25.0, -8e10, 1, 3.0, 7.0, 0.0]])
+    p = cases.scaled_params(nx, ny, steps=40, lat=72.0, Gaussians=G)
+    p.hypernu[0] = nu2
+    w, o, s, fs, fo = run_both(p, cases.single_bump(nx + ny), 40)
+    assert s.fused
+    assert fs == fo and (fo == 0 or fo > 10), (fs, fo)
+    assert np.nanmax(np.abs(w.field("tracer"))) > 0.05  # NaN where the unstable run blew up
+    assert_same(w, o, f"fused storms {nx}x{ny} nu2={nu2}")
+
+
 def test_fused_storm_guards():
     """A check failing in a storm step still sees that step's injection
     (SW2 injects before it checks); a CFL failure stops the step before
