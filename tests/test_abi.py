@@ -104,6 +104,26 @@ def test_reads_reference_case_file(golden):
     assert list(g) == [0.0, 604800.0, 0.0, 45.0, -22.5, 0.0, -3.6, 4.52e11, 0.0, 3.5e6, 3.5e6, 0.0]
 
 
+def test_reads_saturn_smooth_injection_case(golden):
+    """The reference's SaturnSmoothInjection case: macros from define(),
+    <expr> times, the planet include and the 861-row Cassini wind table
+    (the case file's own nowind include is commented out)."""
+    cwd = os.getcwd()
+    os.chdir(os.path.join(golden, "saturn"))
+    try:
+        p = sw2_amd.read_case("minicanal.sw")
+    finally:
+        os.chdir(cwd)
+    assert (p.sch, p.polar, p.nx, p.ny) == (1, 0, 600, 100)
+    assert (p.lon0_deg, p.lon1_deg, p.lat0_deg, p.lat1_deg) == (0.0, 60.0, 65.0, 75.0)
+    assert (p.t0, p.t1, p.Dt, p.CFL, p.Tau) == (0.0, 6000.0, 60.0, 0.25, 1e9)
+    assert list(p.hypernu) == [0.0, 0.0, 0.0] and p.depth == 500.0
+    assert p.num_vortices == 0 and p.num_Gaussians == 1 and p.wind_rows == 861
+    g = np.ctypeslib.as_array(p.Gaussians, shape=(12,))
+    # <20*DT> = 1200 s, P1 = 68.8, P3 = 1.5e9 (define()), radius and sigma in metres, no tracer
+    assert list(g) == [0.0, 1200.0, 200.0, 30.0, 68.8, 0.0, 14.2, 1.5e9, 0.0, 114e3, 114e3, 0.0]
+
+
 def test_case_file_macro_subset(tmp_path):
     (tmp_path / "inc.txt").write_text("rE 7e7 # radius\nrP 7e7\n")
     txt = cases.case_text(cases.scaled_params(8, 6, steps=2), "x")[0]

@@ -171,3 +171,34 @@ def test_mixed_eta_only_storms_against_fp64():
     s0.step(300)
     s0.download()
     assert errors(w64.interior("eta"), w0.interior("eta"))[0] > 100 * TOL_L2
+
+
+def test_mixed_saturn_smooth_injection_case(golden):
+    """The reference's SaturnSmoothInjection case (its storm injects height
+    only) in the mixed form against the FP64 GPU run of the same case."""
+    import os
+    import sw2_amd
+    cwd = os.getcwd()
+    os.chdir(os.path.join(golden, "saturn"))
+    try:
+        p = sw2_amd.read_case("minicanal.sw")
+    finally:
+        os.chdir(cwd)
+    o = Oracle(p)
+    o.preloop()
+    runs = {}
+    for form in (FORM_FP64, FORM_MIXED):
+        w = World(p)
+        for f in ("eta", "u", "v", "tracer", "perturbs", "windx", "windy"):
+            w.field(f)[:] = o.field(f)
+        s = Session(w, form)
+        assert s.form == form and s.fused
+        s.begin(0, p.t0)
+        s.step(100)
+        s.download()
+        runs[form] = (w, s.failed_step()[0])
+    (w64, f64), (w32, f32) = runs[FORM_FP64], runs[FORM_MIXED]
+    assert f64 == f32 == 0
+    e = report("mixed minicanal.sw 100 steps", w32, w64.interior)
+    for f, (l2, mx) in e.items():
+        assert l2 <= TOL_L2, (f, l2)

@@ -221,6 +221,34 @@ def test_jupiter_energy_case(golden):
     assert w.energy() == o.energy()
 
 
+def test_saturn_smooth_injection_case(golden):
+    """The reference's SaturnSmoothInjection case (600x100 over 60 deg of
+    longitude, superbee, no hyperviscosity, the Cassini wind table and a
+    moving storm injecting height for its first 20 steps) through its whole
+    run of 100 steps on the fused step, value for value."""
+    cwd = os.getcwd()
+    os.chdir(os.path.join(golden, "saturn"))
+    try:
+        p = sw2_amd.read_case("minicanal.sw")
+    finally:
+        os.chdir(cwd)
+    w = World(p)
+    o = Oracle(p)
+    o.preloop()
+    for f in FIELDS + ("windx", "windy"):
+        w.field(f)[:] = o.field(f)
+    s = Session(w)
+    assert s.fused
+    s.begin(0, p.t0)
+    s.step(100)
+    s.download()
+    fo, _ = o.step(100)
+    assert s.failed_step()[0] == fo == 0
+    assert_same(w, o, "minicanal.sw")
+    assert w.energy() == o.energy()
+    assert w.mass() == o.mass()
+
+
 def test_cfl_guard_fails_on_the_same_step():
     p = cases.scaled_params(40, 20, steps=10, lat=70.0, Dt=50000.0, CFL=0.25)
     w, o, s, fs, fo = run_both(p, cases.single_bump(7), 10)
