@@ -1,6 +1,7 @@
 """The drop-in boundary end to end: SW2's driver program (sw2_amd/SW, the
 library's build of main/Main.c) runs the reference's own JupiterEnergyTest
-case file through initialize / loop / destroy on the GPU.  Its printed
+and SaturnSmoothInjection case files through initialize / loop / destroy on
+the GPU.  Its printed
 ENERGY lines and its last frame must be those of the oracle's run of the
 same case (core.c:134-338, energy.c:25-55, io.c:27-76)."""
 import os
@@ -65,9 +66,50 @@ def test_sw_driver_runs_reference_case(golden, tmp_path):
         assert np.array_equal(a, b), (f, np.argwhere(a != b)[:3])
 
 
-def ctypes_load(world, frame, folder):
+def ctypes_load(world, frame, folder, name=b"jupiters"):
     from ctypes import byref, c_double, c_int
     n, t = c_int(), c_double()
-    rc = sw2_amd.lib().load_binary_sw(frame, byref(n), byref(t), b"jupiters", folder.encode(), byref(world.sw))
+    rc = sw2_amd.lib().load_binary_sw(frame, byref(n), byref(t), name, folder.encode(), byref(world.sw))
     assert rc == 0
     return n.value, t.value
+
+
+def test_sw_driver_runs_saturn_case(golden, tmp_path):
+    """The reference's SaturnSmoothInjection case (600 columns: the fused
+    step, with its storm injected after the momentum kernel) through the
+    driver: an ENERGY line every step (IteInfo 1), as the oracle's run, and
+    the last frame's fields.  Frames are saved every 50 steps instead of
+    every step."""
+    src = os.path.join(golden, "saturn")
+    for f in ("SaturnEGM2.planet", "Saturn_Cassini.wind", "nowind.wind"):
+        shutil.copy(os.path.join(src, f), tmp_path / f)
+    txt = open(os.path.join(src, "minicanal.sw")).read()
+    txt = re.sub(r"(?m)^SaveEvery .*$", "SaveEvery 50", txt)
+    (tmp_path / "minicanal.sw").write_text(txt)
+    print("SW 1 1 minicanal (100 steps)", flush=True)
+    r = subprocess.run([os.path.join(ROOT, "sw2_amd", "SW"), "1", "1", "minicanal", "out"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "PROGRAM ENDED" in r.stdout
+    printed = [line.split()[1:] for line in r.stdout.splitlines() if line.startswith("ENERGY ")]
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        p = sw2_amd.read_case("minicanal.sw")
+    finally:
+        os.chdir(cwd)
+    o = Oracle(p)
+    o.preloop()
+    ke0, ape0 = o.energy()
+    fo, rec = o.step(100, energy=True)
+    assert fo == 0
+    expect = [(p.t0, ke0, ape0)] + [tuple(x) for x in rec]
+    assert len(printed) == len(expect), (len(printed), len(expect))
+    for got, (t, ke, ape) in zip(printed, expect):
+        assert got == ["%e" % t, "%e" % ke, "%e" % ape, "%e" % (ke + ape)]
+    w = sw2_amd.World(p)
+    n, t = ctypes_load(w, 2, str(tmp_path / "out"), b"minicanal")
+    assert n == 100 and t == o.time
+    for f in FIELDS:
+        a, b = w.interior(f), o.interior(f)
+        assert np.array_equal(a, b), (f, np.argwhere(a != b)[:3])
