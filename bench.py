@@ -34,7 +34,10 @@ Prints one JSON line:
                 from its measured per-size step times;
   served        Poisson arrivals at 4 requests/s of storm runs (512x256 ..
                 4096x2048, 500..20000 steps) through the continuous-batching
-                loop: p50 / p99 time-to-completion and runs/hour.
+                loop: p50 / p99 time-to-completion and runs/hour, against
+                the FP64 form serving the same stream one run at a time
+                (first come, first served, from its measured per-size step
+                times).
 """
 import argparse
 import json
@@ -271,6 +274,16 @@ def bench_served(n):
     srv.close()
     st.update({"arrivals_per_s": 4.0, "capacity": 48, "hbm_budget_bytes": 64e9, "tick_steps": 40,
                "sizes": "512x256/1024x512/2048x1024/4096x2048 at 0.4/0.3/0.2/0.1", "steps": "uniform 500..20000"})
+    # the FP64 form serving the same stream one run at a time, first come
+    # first served, from its measured step time per size
+    progress("served stream: FP64 step times per request size")
+    ms64 = {sz: fp64_step_ms(*sz) for sz in sorted({(r[1], r[2]) for r in reqs})}
+    t = 0.0
+    for ta, nx, ny, steps, *_ in sorted(reqs, key=lambda r: r[0]):
+        t = max(t, ta) + steps * ms64[(nx, ny)] / 1e3
+    st.update({"fp64_fcfs_wall_s": t, "fp64_runs_per_hour": len(reqs) / t * 3600.0,
+               "vs_fp64_runs_per_hour": st["runs_per_hour"] / (len(reqs) / t * 3600.0),
+               "fp64_ms_per_step": {f"{a}x{b_}": v for (a, b_), v in ms64.items()}})
     return st
 
 

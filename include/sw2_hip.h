@@ -317,8 +317,9 @@ int sw2_batch_get_launch(sw2_batch *b, int *blocks_eta, int *blocks_uv, int *ste
  * step); returns without waiting. */
 int sw2_batch_step(sw2_batch *b, int nsteps);
 int sw2_batch_sync(sw2_batch *b);
-/* Hold a member (paused != 0: its tiles are skipped and its step count
- * stays) or release it, between steps; a scheduler's knob. */
+/* Hold a member (paused != 0: its tiles leave the launches and its step
+ * count stays) or release it, between steps; a scheduler's knob.  With one
+ * workgroup per tile a change re-captures the step graph at the next step. */
 int sw2_batch_pause(sw2_batch *b, int id, int paused);
 /* Steps completed, whether the member's run ended, first failed step. */
 int sw2_batch_member(sw2_batch *b, int id, int *steps_done, int *done, int *failed_step);

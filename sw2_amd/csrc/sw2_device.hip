@@ -1532,11 +1532,15 @@ extern "C" int sw2_batch_create(int form, int capacity, double hbm_budget_bytes,
     return 0;
 }
 
+// number the tiles of the members that advance: a held member keeps its
+// slot but gets no tiles (an empty span at its tile0, which the kernels'
+// slot search never returns), so a launch with one workgroup per tile has
+// no workgroups for held runs
 static void batch_retile(sw2_batch *b) {
     int t = 0;
     for (int m = 0; m < b->cap; m++) {
         b->h_mem[m].tile0 = t;
-        t += b->h_mem[m].ntx * b->h_mem[m].nty;
+        if (!b->paused[m]) t += b->h_mem[m].ntx * b->h_mem[m].nty;
     }
     // a graph launching one workgroup per tile holds the old tile count
     if (t != b->h_hdr.total_tiles && (b->blocks_eta == 0 || b->blocks_uv == 0) && b->gexec) {
@@ -1767,8 +1771,8 @@ extern "C" int sw2_batch_member(sw2_batch *b, int id, int *steps_done, int *done
     return 0;
 }
 
-// hold a member (its tiles skipped, its step count kept) or release it;
-// between steps only: the caller has synchronised the batch
+// hold a member (its tiles taken out of the launches, its step count kept)
+// or release it; between steps only: the caller has synchronised the batch
 extern "C" int sw2_batch_pause(sw2_batch *b, int id, int paused) {
     if (id < 0 || id >= b->cap || !b->sess[id]) {
         sw2_set_error("no member %d", id);
@@ -1778,7 +1782,10 @@ extern "C" int sw2_batch_pause(sw2_batch *b, int id, int paused) {
     HIPCHECK(hipMemcpyAsync(reinterpret_cast<char *>(b->d_st + id) + offsetof(BState, paused), &v, sizeof v,
                             hipMemcpyHostToDevice, b->st));
     HIPCHECK(hipStreamSynchronize(b->st));
-    b->paused[id] = v;
+    if (b->paused[id] != v) {
+        b->paused[id] = v;
+        batch_retile(b);
+    }
     return 0;
 }
 
