@@ -1744,7 +1744,8 @@ extern "C" int sw2_batch_step(sw2_batch *b, int nsteps) {
         HIPCHECK(hipGetLastError());
         return 0;
     }
-    if (batch_capture(b)) return -1;
+    // the graph is captured only when this call replays it
+    if (nsteps >= b->gsteps && batch_capture(b)) return -1;
     int q = 0;
     for (; q + b->gsteps <= nsteps; q += b->gsteps) HIPCHECK(hipGraphLaunch(b->gexec, b->st));
     for (; q < nsteps; q++) batch_launch_step(b);
