@@ -152,3 +152,38 @@ def test_batch_members_with_storms_equal_solo_runs():
         assert np.array_equal(w.field("perturbs"), ref.field("perturbs"))
     assert b.download(members[0][0]).field("perturbs").max() == 1.0
     b.close()
+
+
+def test_all_members_held_then_released():
+    """Every member held: the launches number no tiles, and the steps
+    change nothing.  Released, one member mid-table held for a while (its
+    tiles out of the numbering, its neighbours' renumbered), every member
+    ends on its solo result, through graph replays and direct launches."""
+    members = small_members(11, 5)
+    b = Batch(capacity=6)
+    b.set_launch(0, 0, 4)
+    ids = []
+    for p, bumps, steps in members:
+        w, _ = cases.make_world(p, bumps)
+        ids.append(b.add(w, steps))
+    b.step(6)
+    b.sync()
+    for mid in ids:
+        b.pause(mid)
+    b.step(9)
+    b.sync()
+    assert all(b.member(mid)["steps"] == 6 for mid in ids)
+    for mid in ids:
+        b.pause(mid, False)
+    b.pause(ids[2])
+    b.step(10)
+    b.sync()
+    assert b.member(ids[2])["steps"] == 6 and b.member(ids[0])["steps"] == 16
+    b.pause(ids[2], False)
+    b.step(max(m[2] for m in members) + 5)
+    b.sync()
+    for mid, (p, bumps, steps) in zip(ids, members):
+        st = b.member(mid)
+        assert st["done"] and st["steps"] == steps and st["failed_step"] == 0, st
+        same(b.download(mid), solo(p, bumps, steps), f"member {mid}")
+    b.close()
