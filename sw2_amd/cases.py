@@ -159,12 +159,33 @@ def make_world(params, bumps):
 
 
 # ------------------------------------------------------------ files
-def case_text(p, bname):
+def params_dict(p):
+    """The values of Params p as case_text writes them: the keyword
+    dictionary of build_params, or for Params read from a case file
+    (read_case) the struct's fields and copies of its tables."""
+    if hasattr(p, "_dict"):
+        return p._dict
+    d = {n: getattr(p, n) for n, _ in p._fields_ if n not in ("hypernu", "vortices", "Gaussians", "wind")}
+    d["hypernu"] = tuple(p.hypernu)
+    for name, cnt, ncol in (("vortices", "num_vortices", 7), ("Gaussians", "num_Gaussians", 12), ("wind", "wind_rows", 2)):
+        n = getattr(p, cnt)
+        d[name] = np.ctypeslib.as_array(getattr(p, name), shape=(n * ncol,)).reshape(n, ncol).copy() if n else None
+    return d
+
+
+def _table(rows, ncol, label):
+    rows = np.zeros((0, ncol)) if rows is None else np.asarray(rows, dtype=float).reshape(-1, ncol)
+    return [f"{len(rows)} # {label} rows", f"{ncol}"] + [" ".join(repr(float(x)) for x in r) for r in rows]
+
+
+def case_text(p, bname, flat=False):
     """SW2 case file (tag order of init.c:48-229) for Params p, and the
-    planet file it includes."""
-    d = p._dict
+    planet file it includes (flat: the planet's tags written in place and
+    no m4 construct at all, so the file reads without pre-processing)."""
+    d = params_dict(p)
     hn = d["hypernu"]
-    wind = np.asarray(d.get("wind", ZERO_WIND))
+    wind = np.asarray(ZERO_WIND if d.get("wind") is None else d["wind"])
+    planet_lines = [f"Omega {d['Omega']!r}", f"rE {d['rE']!r}", f"rP {d['rP']!r}", f"gE {d['gE']!r}"]
     lines = [
         f"# {bname}: jet as background wind, storm bump(s) in frame 1",
         f"sch {d['sch']}", "polar 0",
@@ -173,12 +194,13 @@ def case_text(p, bname):
         f"t0 {d['t0']!r}", f"t1 {d['t1']!r}", f"Dt {d['Dt']!r}", f"CFL {d['CFL']!r}", f"Tau {d['Tau']!r}",
         f"Hypernu {hn[0]!r} {hn[1]!r} {hn[2]!r}",
         f"IteInfo {d['IteInfo']}", f"SaveEvery {d['SaveEvery']}", f"LoadFrom {d['LoadFrom']}", "reftype 0",
-        f"include({bname}.planet)",
-        f"depth {d['depth']!r}", "geoeq 0",
-        "0 # vortex rows", "7", "0 # Gaussian rows", "12", "TracerDis 0",
+    ] + (planet_lines if flat else [f"include({bname}.planet)"]) + [
+        f"depth {d['depth']!r}", f"geoeq {d.get('geoeq', 0)}",
+    ] + _table(d.get("vortices"), 7, "vortex") + _table(d.get("Gaussians"), 12, "Gaussian") + [
+        f"TracerDis {float(d.get('TracerDis', 0.0))!r}",
         f"{len(wind)} # wind rows", "2",
     ] + [f"{float(a)!r} {float(b)!r}" for a, b in wind]
-    planet = "\n".join([f"Omega {d['Omega']!r}", f"rE {d['rE']!r}", f"rP {d['rP']!r}", f"gE {d['gE']!r}"]) + "\n"
+    planet = "\n".join(planet_lines) + "\n"
     return "\n".join(lines) + "\n", planet
 
 

@@ -10,9 +10,10 @@ from sw2_amd import FORM_FP64, FORM_MIXED, Session, SW2Error, World, cases
 
 pytestmark = pytest.mark.gpu
 
-# the mixed form's tolerance on eta, u and v: relative L2 error against the
-# FP64 result (||a - b||_2 / ||b||_2 over the interior)
-TOL_L2 = 1e-5
+# the mixed form's tolerance on eta, u and v against the FP64 result, both
+# relative L2 (||a - b||_2 / ||b||_2) and relative max (max|a - b| / max|b|)
+# over the interior
+TOL = 1e-5
 
 
 def errors(a, b):
@@ -54,8 +55,7 @@ def test_mixed_config0_1000_steps():
     assert fs == fo == 0
     e = report("mixed c0 1000 steps", w, o.interior)
     for f, (l2, mx) in e.items():
-        assert l2 <= TOL_L2, (f, l2)
-        assert mx <= 10 * TOL_L2, (f, mx)
+        assert l2 <= TOL and mx <= TOL, (f, l2, mx)
 
 
 @pytest.mark.parametrize("sch", list(range(10)))
@@ -76,28 +76,30 @@ def test_mixed_all_limiters(sch):
     assert fs == fo == 0
     e = report(f"mixed sch={sch} 128x24 {n} steps", w, o.interior)
     for f, (l2, mx) in e.items():
-        assert l2 <= TOL_L2, (sch, f, l2)
+        assert l2 <= TOL and mx <= TOL, (sch, f, l2, mx)
 
 
 def test_mixed_against_fp64_gpu_at_4096x2048():
-    """The 4096x2048 workload, 200 steps: mixed against the FP64 GPU form
-    (which the parity tests hold bit-equal to the oracle)."""
-    p = cases.config_params("c1", steps=200)
+    """The mixed-precision workload at its size and step count (4096x2048,
+    5000 steps): mixed against the FP64 GPU form, which the full-size tests
+    hold bit-equal to the oracle."""
+    n = 5000
+    p = cases.config_params("c2", steps=n)
     bumps = cases.single_bump(2024)
     w64, _ = cases.make_world(p, bumps)
     s64 = Session(w64, FORM_FP64)
     s64.begin(0, p.t0)
-    s64.step(200)
+    s64.step(n)
     s64.download()
     w32, _ = cases.make_world(p, bumps)
     s32 = Session(w32, FORM_MIXED)
     s32.begin(0, p.t0)
-    s32.step(200)
+    s32.step(n)
     s32.download()
     assert s64.failed_step()[0] == s32.failed_step()[0] == 0
-    e = report("mixed c1 200 steps", w32, w64.interior)
+    e = report(f"mixed c2 {n} steps", w32, w64.interior)
     for f, (l2, mx) in e.items():
-        assert l2 <= TOL_L2, (f, l2)
+        assert l2 <= TOL and mx <= TOL, (f, l2, mx)
     ke64, ape64, m64 = s64.energy()
     ke32, ape32, m32 = s32.energy()
     print(f"energy fp64 {ke64:.6e} {ape64:.6e} mass {m64:.12e}; mixed {ke32:.6e} {ape32:.6e} mass {m32:.12e}", flush=True)
@@ -163,14 +165,14 @@ def test_mixed_eta_only_storms_against_fp64():
     assert w64.field("perturbs").max() == 1.0
     e = report("mixed eta-only storms 300 steps", w32, w64.interior)
     for f, (l2, mx) in e.items():
-        assert l2 <= TOL_L2, (f, l2)
+        assert l2 <= TOL and mx <= TOL, (f, l2, mx)
     # the storms moved the height: the comparison is not of two quiet runs
     w0, _ = cases.make_world(cases.scaled_params(512, 256, steps=300, lat=75.0), bumps)
     s0 = Session(w0, FORM_FP64)
     s0.begin(0, p.t0)
     s0.step(300)
     s0.download()
-    assert errors(w64.interior("eta"), w0.interior("eta"))[0] > 100 * TOL_L2
+    assert errors(w64.interior("eta"), w0.interior("eta"))[0] > 100 * TOL
 
 
 def test_mixed_saturn_smooth_injection_case(golden):
@@ -201,4 +203,4 @@ def test_mixed_saturn_smooth_injection_case(golden):
     assert f64 == f32 == 0
     e = report("mixed minicanal.sw 100 steps", w32, w64.interior)
     for f, (l2, mx) in e.items():
-        assert l2 <= TOL_L2, (f, l2)
+        assert l2 <= TOL and mx <= TOL, (f, l2, mx)
