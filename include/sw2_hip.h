@@ -268,6 +268,18 @@ int sw2_session_kernel_times(sw2_session *s, int nsteps, double *ms, int nk);
  * SW2's serial order: for the FP64 form the host's calc_energy_balance and
  * sw2_mass give SW2's exact sums). */
 int sw2_session_energy(sw2_session *s, double *ke, double *ape, double *mass);
+/* No counterpart in SW2: sw2_session_energy for callers that cannot
+ * allocate -- code under HIP graph capture, tight serving loops.  The same
+ * reduction (same blocks, same order: the same three sums) of the session's
+ * current state, written to d_out3 (caller-owned device memory, 3 doubles:
+ * KE, APE, mass) on `stream` (a hipStream_t; NULL: the session's stream)
+ * without host synchronisation and without allocating anything: its partial
+ * sums live in a static device array of 3 x 1024 doubles (24 KiB) that every
+ * GPU the library runs on holds until the process ends.  Any grid reduces to
+ * those 1024 partials, so no size is refused.  Calls that overlap on one GPU
+ * share that array and race; order them on one stream.  Call with the
+ * session's device current. */
+int sw2_session_energy_noalloc(sw2_session *s, double *d_out3, void *stream);
 /* Copy eta, u, v, tracer, perturbs (halos included) back into SW's arrays.
  * A mixed-form session writes its FP64 mean plus FP32 anomaly. */
 int sw2_session_download(sw2_session *s, sw *SW);

@@ -84,7 +84,7 @@ EXPORTS = [
     "laplacian_vels", "laplacian_eta",
     "save_binary_sw", "load_binary_sw",
     "sw2_session_create", "sw2_session_begin", "sw2_session_step", "sw2_session_time", "sw2_session_kernel_times",
-    "sw2_session_energy", "sw2_session_download", "sw2_session_form",
+    "sw2_session_energy", "sw2_session_energy_noalloc", "sw2_session_download", "sw2_session_form",
     "sw2_session_failed_step", "sw2_session_last_cfl", "sw2_session_fused", "sw2_session_state", "sw2_session_destroy",
     "sw2_device_cus", "sw2_batch_footprint", "sw2_batch_create", "sw2_batch_add", "sw2_batch_set_launch", "sw2_batch_get_launch",
     "sw2_batch_step", "sw2_batch_sync", "sw2_batch_pause", "sw2_batch_member", "sw2_batch_finished", "sw2_batch_download",
@@ -138,6 +138,7 @@ def lib():
     L.sw2_session_kernel_times.argtypes = [c_void_p, c_int, P, c_int]
     L.sw2_session_download.argtypes = [c_void_p, POINTER(SW)]
     L.sw2_session_energy.argtypes = [c_void_p, P, P, P]
+    L.sw2_session_energy_noalloc.argtypes = [c_void_p, c_void_p, c_void_p]
     L.sw2_session_form.argtypes = [c_void_p]
     L.sw2_session_failed_step.argtypes = [c_void_p, ctypes.c_char_p, c_int]
     L.sw2_session_last_cfl.argtypes = [c_void_p, P]

@@ -616,6 +616,39 @@ __global__ void k_energy_fin(int nb, const double *part, double *out) {
 }
 }  // namespace
 
+// ------------------------------------------------ energy without allocation
+// sw2_session_energy_noalloc keeps the partial sums of the energy reduction
+// (ENERGY_PARTS blocks x KE, APE, mass) in this file-scope array instead of
+// the session's scratch: every device the library runs on holds one copy
+// until the process ends.  In host code `scratch` names a host-side stand-in,
+// never the device copy: the device address comes from hipGetSymbolAddress,
+// taken once per device with that device current and kept in scratch_dev[].
+constexpr int ENERGY_PARTS = 1024;  // blocks of k_energy: any grid reduces to this many partials
+constexpr int MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[3 * ENERGY_PARTS];
+static double *scratch_dev[MAX_DEVICES];
+
+// the device address of `scratch` on the current device (taken on the first
+// call per device; sw2_session_create makes that call, so a caller that
+// captures graphs has it before capture begins)
+static int scratch_address(double **out) {
+    int dev = -1;
+    HIPCHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) {
+        sw2_set_error("device %d: the scratch table holds devices 0..%d", dev, MAX_DEVICES - 1);
+        return -1;
+    }
+    double *p = __atomic_load_n(&scratch_dev[dev], __ATOMIC_ACQUIRE);
+    if (!p) {
+        void *a = nullptr;
+        HIPCHECK(hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)));
+        p = (double *)a;
+        __atomic_store_n(&scratch_dev[dev], p, __ATOMIC_RELEASE);
+    }
+    *out = p;
+    return 0;
+}
+
 // ====================================================================== session
 struct sw2_session {
     sw *SW;
@@ -824,7 +857,11 @@ extern "C" int sw2_session_create(sw *SW, int form, sw2_session **out) {
     if (!rc && hipMemcpy(s->d_tab, tab, sizeof(double) * NTD * R, hipMemcpyHostToDevice) != hipSuccess) rc = -1;
     free(tab);
     s->g.T = s->d_tab;
-    rc |= dalloc(&s->d_part, 3 * 1024 + 3);
+    rc |= dalloc(&s->d_part, 3 * ENERGY_PARTS + 3);
+    {
+        double *sp = nullptr;  // the allocation-free energy's scratch address on this device
+        if (!rc && scratch_address(&sp)) rc = -1;
+    }
     if (form == SW2_FORM_MIXED) {
         if (!rc) rc = mixed_setup(s, SW);
         if (!rc && hipMalloc((void **)&s->fl, sizeof(Flags)) != hipSuccess) rc = -1;
@@ -1313,7 +1350,7 @@ extern "C" int sw2_session_kernel_times(sw2_session *s, int nsteps, double *ms, 
 }
 
 extern "C" int sw2_session_energy(sw2_session *s, double *ke, double *ape, double *mass) {
-    const int nb = 1024;
+    const int nb = ENERGY_PARTS;
     double *out = s->d_part + 3 * nb;
     if (s->form == SW2_FORM_MIXED)
         k_energy<float><<<nb, 256, 0, s->st>>>(s->g, s->m_eta, s->m_u, s->m_v, s->d_mt + MT_HBAR * s->g.R, s->depth, s->d_part);
@@ -1327,6 +1364,26 @@ extern "C" int sw2_session_energy(sw2_session *s, double *ke, double *ape, doubl
     if (ke) *ke = h[0];
     if (ape) *ape = h[1];
     if (mass) *mass = h[2];
+    return 0;
+}
+
+extern "C" int sw2_session_energy_noalloc(sw2_session *s, double *d_out3, void *stream) {
+    if (!s || !d_out3) {
+        sw2_set_error("sw2_session_energy_noalloc: session and device output needed");
+        return -1;
+    }
+    const int nb = ENERGY_PARTS;  // the same reduction tree as sw2_session_energy
+    double *part = nullptr;
+    if (scratch_address(&part)) return -1;
+    hipStream_t st = stream ? (hipStream_t)stream : s->st;
+    // every block writes its own partials and k_energy_fin writes all three
+    // outputs, so nothing in the scratch has to start at zero
+    if (s->form == SW2_FORM_MIXED)
+        k_energy<float><<<nb, 256, 0, st>>>(s->g, s->m_eta, s->m_u, s->m_v, s->d_mt + MT_HBAR * s->g.R, s->depth, part);
+    else
+        k_energy<double><<<nb, 256, 0, st>>>(s->g, s->eta, s->u, s->v, nullptr, s->depth, part);
+    k_energy_fin<<<1, 64, 0, st>>>(nb, part, d_out3);
+    HIPCHECK(hipGetLastError());
     return 0;
 }
 
