@@ -28,11 +28,13 @@ extern "C" {
 #endif
 
 /* ---------------------------------------------------------------------------
- * sppde-compatible map and field access (sppde/sppde.h:46-67, 131-142).
- * Only the single-map case is meaningful here: one rank owns the whole
- * domain, x is periodic, y is walled (channel).
+ * sppde map and field access (sppde/sppde.h:44-67, 131-142), field for field
+ * as SW2 declares it, so that code compiled against SW2's own headers reads
+ * every member of map and sw at the offsets this library writes.  Only one
+ * map is ever in use: one GPU owns the whole domain, x is periodic, y is
+ * walled (channel).  The per-process tables al0 / al1 / pco hold entry 0.
  * ------------------------------------------------------------------------- */
-#define SW2_MAXP 1 /* one GPU, one map */
+#define SW2_MAXP 1920 /* sppde.h:44 (MAXP): slots of the per-process tables */
 
 typedef struct {
     int sh;        /* halo width (2) */
@@ -41,11 +43,15 @@ typedef struct {
     int l1[4];     /* last owned index per axis */
     int gl0[4];    /* global first index */
     int gl1[4];    /* global last index */
+    int al0[SW2_MAXP][4]; /* first index per process and axis (process 0 only) */
+    int al1[SW2_MAXP][4]; /* last index per process and axis */
+    int pco[SW2_MAXP][4]; /* processor coordinates per process */
     int s[4];      /* extent per axis including halos */
     int S;         /* number of doubles in a field */
     int gpc[4];    /* processors per axis (all 1) */
     int pc[4];     /* my processor coordinates (all 1) */
-    int nb_e, nb_w, nb_n, nb_s; /* neighbours (unused: single map) */
+    int nb_e, nb_w, nb_n, nb_s; /* neighbours: 0 (myself) along periodic x, -1 along y */
+    int nb_t, nb_b;             /* top / bottom (3-D maps only): -1 */
     int bc0[4];    /* do I own the initial boundary of axis a (0 if periodic) */
     int bc1[4];    /* do I own the final boundary of axis a (0 if periodic) */
     int per[4];    /* periodicity per axis */

@@ -16,14 +16,18 @@ LIB_PATH = os.path.join(_HERE, "libsw2.so")
 P = POINTER(c_double)
 
 
+MAXP = 1920  # per-process table slots of the sppde map (sppde/sppde.h:44)
+
+
 class Map(ctypes.Structure):
-    """sppde map, single process (sppde/sppde.h:46-67)."""
+    """sppde map (sppde/sppde.h:46-67), one process."""
 
     _fields_ = [
         ("sh", c_int), ("nd", c_int),
         ("l0", c_int * 4), ("l1", c_int * 4), ("gl0", c_int * 4), ("gl1", c_int * 4),
+        ("al0", (c_int * 4) * MAXP), ("al1", (c_int * 4) * MAXP), ("pco", (c_int * 4) * MAXP),
         ("s", c_int * 4), ("S", c_int), ("gpc", c_int * 4), ("pc", c_int * 4),
-        ("nb_e", c_int), ("nb_w", c_int), ("nb_n", c_int), ("nb_s", c_int),
+        ("nb_e", c_int), ("nb_w", c_int), ("nb_n", c_int), ("nb_s", c_int), ("nb_t", c_int), ("nb_b", c_int),
         ("bc0", c_int * 4), ("bc1", c_int * 4), ("per", c_int * 4),
     ]
 

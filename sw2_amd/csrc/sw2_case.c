@@ -17,6 +17,7 @@
 #include <limits.h>
 #include <math.h>
 #include <stdarg.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -37,29 +38,36 @@ typedef struct {
     int err;
 } xstate;
 
+/* fac(a): the factorial of a's integer part, as tinyexpr's fac gives it:
+ * NaN below zero, infinity once the value no longer fits in 64-bit unsigned
+ * integers (from 21!).  The running product 2*3*...*k is exact in double up
+ * to 20!: with its factors of two set aside, 20! has 44 significant bits. */
 static double x_fac(double a) {
     if (a < 0.0) return NAN;
-    if (a > UINT_MAX) return INFINITY;
-    unsigned int ua = (unsigned int)(a);
-    unsigned long int result = 1, i;
-    for (i = 1; i <= ua; i++) {
-        if (i > ULONG_MAX / result) return INFINITY;
-        result *= i;
-    }
-    return (double)result;
+    if (!(a < 21.0)) return INFINITY;
+    const int n = (int)a;
+    double f = 1.0;
+    for (int k = 2; k <= n; k++) f *= (double)k;
+    return f;
 }
+
+/* ncr(n, r): the binomial coefficient of the integer parts, as tinyexpr's
+ * ncr: NaN for a negative argument or n < r, infinity for arguments beyond
+ * 32 bits or a value beyond 64 bits.  C(m, k) with k = min(r, m - r) is
+ * built up along the diagonal C(m-k+i, i) = C(m-k+i-1, i-1) * (m-k+i) / i,
+ * each step an exact division in 128-bit arithmetic. */
 static double x_ncr(double n, double r) {
     if (n < 0.0 || r < 0.0 || n < r) return NAN;
-    if (n > UINT_MAX || r > UINT_MAX) return INFINITY;
-    unsigned long int un = (unsigned int)(n), ur = (unsigned int)(r), i;
-    unsigned long int result = 1;
-    if (ur > un / 2) ur = un - ur;
-    for (i = 1; i <= ur; i++) {
-        if (result > ULONG_MAX / (un - ur + i)) return INFINITY;
-        result *= un - ur + i;
-        result /= i;
+    if (n > 4294967295.0 || r > 4294967295.0) return INFINITY;
+    const uint64_t m = (uint64_t)n;
+    uint64_t k = (uint64_t)r;
+    if (k > m - k) k = m - k;
+    unsigned __int128 c = 1;
+    for (uint64_t i = 1; i <= k; i++) {
+        c = c * (m - k + i) / i;
+        if (c > (unsigned __int128)UINT64_MAX) return INFINITY;
     }
-    return (double)result;
+    return (double)(uint64_t)c;
 }
 static double x_pi(void) { return 3.14159265358979323846; }
 static double x_e(void) { return 2.71828182845904523536; }
@@ -399,60 +407,75 @@ static int cleanline(char *q) { /* cleanstr: cut at '\n' or '#', report non-blan
     return text;
 }
 
-static int evalline(lreader *r, char *input) { /* evalstr */
-    char result[4096], expr[4096];
-    char *q = input;
-    int copy = 0, p = 0, pr = 0, backs = 0, skip = 0;
+typedef struct {
+    char *buf;
+    size_t n, cap;
+} sbuf;
+
+static void sb_add(sbuf *b, const char *s, size_t n) {
+    if (b->n + n >= b->cap) n = b->cap - 1 - b->n; /* lines are bounded: truncate as SW2's fixed buffers would */
+    memcpy(b->buf + b->n, s, n);
+    b->n += n;
+    b->buf[b->n] = '\0';
+}
+
+/* The value of every <expr> on a line, written back in its place with
+ * %.15e (what SW2's reader does before it scans a line,
+ * sppde_parser.c:84-143).  The line is taken as a run of segments:
+ *   text        copied as it stands (a backslash before any other character
+ *               stays; a backslash that ends the line is dropped);
+ *   <expr>      evaluated and replaced by its value; no backslash inside;
+ *   \< or \>   the bracket itself, and from there on the line is text in
+ *               which only these two escapes are still read (the bracket
+ *               kept, the backslash dropped): SW2's way of writing a
+ *               literal '<' or '>'.
+ * A '<' without its '>', or a '>' outside any <expr>, is an error. */
+static int subst_exprs(lreader *r, char *line) {
+    char outb[4096];
+    sbuf out = {outb, 0, sizeof outb};
+    outb[0] = '\0';
+    int literal = 0; /* after the first escaped bracket */
+    const char *q = line;
     while (*q) {
-        if (backs == 1) {
-            if (*q == '<' || *q == '>') {
-                skip = 1;
-                backs = 0;
-            }
-        }
-        if (*q == '\\') {
-            backs = 1;
-            q++;
+        const int esc = q[0] == '\\' && (q[1] == '<' || q[1] == '>');
+        if (esc) {
+            sb_add(&out, q + 1, 1);
+            q += 2;
+            literal = 1;
             continue;
         }
-        if (*q == '>' && skip == 0) {
-            copy = 0;
-            expr[p] = '\0';
-            int e;
-            double v = sw2_eval_expr(expr, &e);
-            if (e) {
-                snprintf(r->err, sizeof r->err, "Expression='%s' can not be evaluated", expr);
+        if (q[0] == '\\' && q[1] == '\0') break;
+        if (!literal && *q == '<') {
+            const char *close = strchr(q + 1, '>');
+            if (!close) {
+                snprintf(r->err, sizeof r->err, "missing '>' while evaluating '%s'", line);
+                return -1;
+            }
+            char ex[4096];
+            size_t len = (size_t)(close - (q + 1));
+            if (len >= sizeof ex) len = sizeof ex - 1;
+            memcpy(ex, q + 1, len);
+            ex[len] = '\0';
+            int bad = strchr(ex, '\\') != NULL;
+            double v = bad ? NAN : sw2_eval_expr(ex, &bad);
+            if (bad) {
+                snprintf(r->err, sizeof r->err, "Expression='%s' can not be evaluated", ex);
                 return -1;
             }
             char num[64];
-            int nch = snprintf(num, sizeof num, "%.15e", v);
-            for (int k = 0; k < nch && pr < 4095; k++) result[pr++] = num[k];
-            q++;
+            int k = snprintf(num, sizeof num, "%.15e", v);
+            sb_add(&out, num, (size_t)k);
+            q = close + 1;
             continue;
         }
-        if (*q == '<' && skip == 0) {
-            copy = 1;
-            p = 0;
-            q++;
-            continue;
+        if (!literal && *q == '>') {
+            snprintf(r->err, sizeof r->err, "'>' without its '<' in '%s'", line);
+            return -1;
         }
-        if (copy) {
-            if (p < 4095) expr[p++] = *q;
-        } else {
-            if (backs == 1) {
-                if (pr < 4095) result[pr++] = '\\';
-                backs = 0;
-            }
-            if (pr < 4095) result[pr++] = *q;
-        }
+        sb_add(&out, q, 1);
         q++;
     }
-    if (copy) {
-        snprintf(r->err, sizeof r->err, "missing '>' while evaluating '%s'", input);
-        return -1;
-    }
-    result[pr] = '\0';
-    strcpy(input, result);
+    memcpy(line, outb, out.n + 1);
     return 0;
 }
 
@@ -465,7 +488,7 @@ static char *nextline(lreader *r) {
         r->line[n] = '\0';
         r->cur = e ? e + 1 : r->cur + strlen(r->cur);
         if (cleanline(r->line)) {
-            if (evalline(r, r->line)) return NULL;
+            if (subst_exprs(r, r->line)) return NULL;
             return r->line;
         }
     }
@@ -473,66 +496,83 @@ static char *nextline(lreader *r) {
     return NULL;
 }
 
-static char *tagline(lreader *r, const char *tag) {
+/* A line as SW2's tag readers take it (sppde_parser.c:161-317): the next
+ * line with text once comments are cut and <expr> values written in, split
+ * into words at blanks and tabs.  With a tag, the first word must be that
+ * tag; the words after it are the values. */
+typedef struct {
+    char *w[128];
+    int n;
+} words;
+
+static int next_words(lreader *r, const char *tag, words *W) {
     char *l = nextline(r);
-    if (!l) return NULL;
-    if (!tag[0]) return l;
-    while (*l == ' ' || *l == '\t') l++;
-    char first[256];
-    int k = 0;
-    while (*l && *l != ' ' && *l != '\t' && k < 255) first[k++] = *l++;
-    first[k] = '\0';
-    if (strcmp(first, tag) != 0) {
-        snprintf(r->err, sizeof r->err, "expecting '%s' got '%s'", tag, first);
-        return NULL;
+    if (!l) return -1;
+    char *save = NULL;
+    W->n = 0;
+    for (char *t = strtok_r(l, " \t", &save); t && W->n < 128; t = strtok_r(NULL, " \t", &save)) W->w[W->n++] = t;
+    if (!tag[0]) return 0;
+    if (W->n == 0 || strcmp(W->w[0], tag) != 0) {
+        snprintf(r->err, sizeof r->err, "expecting '%s' got '%s'", tag, W->n ? W->w[0] : "");
+        return -1;
     }
-    return l;
+    memmove(W->w, W->w + 1, sizeof(char *) * (size_t)(W->n - 1));
+    W->n--;
+    return 0;
+}
+
+/* the number a word starts with (what %le reads from it), or -1 */
+static int word_number(const char *w, double *v) {
+    char *end = NULL;
+    *v = strtod(w, &end);
+    return end == w ? -1 : 0;
 }
 
 static int getdouble(lreader *r, const char *tag, double *out) {
-    char *l = tagline(r, tag);
-    if (!l) return -1;
-    if (sscanf(l, "%le", out) != 1) {
-        snprintf(r->err, sizeof r->err, "reading tag '%s' expected valid double and found '%s'", tag, l);
+    words W;
+    if (next_words(r, tag, &W)) return -1;
+    if (W.n == 0 || word_number(W.w[0], out)) {
+        snprintf(r->err, sizeof r->err, "reading tag '%s' expected valid double and found '%s'", tag, W.n ? W.w[0] : "");
         return -1;
     }
     return 0;
 }
 
+/* an integer tag: the value must have no fractional part (SW2 reads it as
+ * a double first, so "1e3" is 1000) */
 static int getint(lreader *r, const char *tag, int *out) {
-    double aux, ip;
-    char *l = tagline(r, tag);
-    if (!l) return -1;
-    if (sscanf(l, "%le", &aux) != 1) {
-        snprintf(r->err, sizeof r->err, "expecting valid integer and found '%s'", l);
+    words W;
+    double v;
+    if (next_words(r, tag, &W)) return -1;
+    if (W.n == 0 || word_number(W.w[0], &v)) {
+        snprintf(r->err, sizeof r->err, "expecting valid integer and found '%s'", W.n ? W.w[0] : "");
         return -1;
     }
-    if (modf(aux, &ip) != 0) {
-        snprintf(r->err, sizeof r->err, "expecting an integer and found '%s'", l);
+    if (v != trunc(v)) {
+        snprintf(r->err, sizeof r->err, "expecting an integer and found '%s'", W.w[0]);
         return -1;
     }
-    *out = (int)aux;
+    *out = (int)v;
     return 0;
 }
 
 static int getndoubles(lreader *r, const char *tag, int nd, double *d) {
-    char *l = tagline(r, tag);
-    if (!l) return -1;
-    char *save = NULL;
+    words W;
+    if (next_words(r, tag, &W)) return -1;
     for (int k = 0; k < nd; k++) {
-        char *tok = strtok_r(k == 0 ? l : NULL, " \t", &save);
-        if (!tok) {
+        if (k >= W.n) {
             snprintf(r->err, sizeof r->err, "expecting more doubles nd=%d i=%d", nd, k + 1);
             return -1;
         }
-        if (sscanf(tok, "%lf", d + k) != 1) {
-            snprintf(r->err, sizeof r->err, "expecting one double and found=<%s>", tok);
+        if (word_number(W.w[k], d + k)) {
+            snprintf(r->err, sizeof r->err, "expecting one double and found=<%s>", W.w[k]);
             return -1;
         }
     }
     return 0;
 }
 
+/* a table: its row count, its column count, then one line per row */
 static int gettable(lreader *r, int *nr, int *nc, double **out) {
     if (getint(r, "", nr) || getint(r, "", nc)) return -1;
     if (*nc > 100 || *nr < 0 || *nc < 0) {
@@ -615,7 +655,8 @@ int sw2_read_case(const char *fname, sw2_params *P) {
     TRY(getint(&r, "LoadFrom", &P->LoadFrom));
     TRY(getint(&r, "reftype", &P->reftype));
     if (P->reftype == 1) {
-        TRY(!tagline(&r, "refname"));
+        words W;
+        TRY(next_words(&r, "refname", &W));
         TRY(gettable(&r, &nr, &nc, &reft));
         free(reft);
         reft = NULL;

@@ -429,7 +429,12 @@ int sw2_all_zero(const double *f, int n) {
 }
 
 /* ------------------------------------------------------------- world set-up */
-static void single_map(sw *SW) { /* createM for one process (sppde_core.c:461-594) */
+/* createM for one process (sppde_core.c:461-594), with the values it gives
+ * there, except the extent of the third axis: SW2 sizes it with five planes
+ * of halo that no 2-D loop reads (s[3] = 6, so every field is six times the
+ * 2-D array); here s[3] = 1 and S is the 2-D field, which is all that
+ * LP / ac and every loop over (i, j) address. */
+static void single_map(sw *SW) {
     map *M = &SW->m;
     memset(M, 0, sizeof *M);
     M->sh = 2;
@@ -438,6 +443,13 @@ static void single_map(sw *SW) { /* createM for one process (sppde_core.c:461-59
     M->l1[1] = M->gl1[1] = SW->nx;
     M->l0[2] = M->gl0[2] = 1;
     M->l1[2] = M->gl1[2] = SW->ny;
+    M->gl0[3] = M->gl1[3] = 1;
+    M->l0[3] = 0;
+    M->l1[3] = 1;
+    M->al0[0][1] = 1, M->al1[0][1] = SW->nx;
+    M->al0[0][2] = 1, M->al1[0][2] = SW->ny;
+    M->al0[0][3] = 0, M->al1[0][3] = 1;
+    M->pco[0][1] = M->pco[0][2] = M->pco[0][3] = 1;
     M->s[1] = SW->nx + 4;
     M->s[2] = SW->ny + 4;
     M->s[3] = 1;
@@ -446,10 +458,12 @@ static void single_map(sw *SW) { /* createM for one process (sppde_core.c:461-59
     M->pc[1] = M->pc[2] = M->pc[3] = 1;
     M->nb_e = M->nb_w = 0; /* periodic x: own neighbour */
     M->nb_n = M->nb_s = -1;
+    M->nb_t = M->nb_b = -1;
     M->per[1] = 1;
     M->per[2] = 0;
     M->bc0[1] = M->bc1[1] = 0; /* periodic axis owns no boundary */
     M->bc0[2] = M->bc1[2] = 1;
+    M->bc0[3] = M->bc1[3] = 1;
     SW->M = M;
 }
 

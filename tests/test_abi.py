@@ -192,3 +192,113 @@ def test_sw_driver_usage_and_rank_check(tmp_path, golden):
     assert "use npx=npy=1" in r.stderr
     assert (tmp_path / "out" / "jupiters.sw.prem4").exists()
     assert (tmp_path / "out" / "jupiters.sw").read_bytes() == open(case + ".sw", "rb").read()
+
+
+REFERENCE = "/root/reference"
+HDR_DRIVER_SRC = os.path.join(ROOT, "tests", "sw2_headers_driver.c")
+
+
+def test_struct_layout_is_sw2s():
+    """sizeof and every offsetof of map and sw, from include/sw2_hip.h and
+    from the ctypes mirror, against SW2's own headers (tests/golden/
+    sw_layout.json, written by tests/golden/make_layout.py)."""
+    import json
+    from sw2_amd._ffi import SW, Map
+    from tests.golden.make_layout import MAP_FIELDS, SW_FIELDS, measure
+    with open(os.path.join(ROOT, "tests", "golden", "sw_layout.json")) as f:
+        ref = json.load(f)
+    ours = measure(['#include "sw2_hip.h"'], [os.path.join(ROOT, "include")])
+    assert ours["map"] == ref["map"] and ours["sw"] == ref["sw"]
+    import ctypes
+    assert ctypes.sizeof(Map) == ref["map"]["sizeof"] and ctypes.sizeof(SW) == ref["sw"]["sizeof"]
+    for f in MAP_FIELDS:
+        assert getattr(Map, f).offset == ref["map"][f], f
+    for f in SW_FIELDS:
+        assert getattr(SW, f).offset == ref["sw"][f], f
+
+
+def build_headers_driver(out):
+    """The caller of tests/sw2_headers_driver.c, compiled against SW2's own
+    sw.h and sppde.h and linked against libsw2.so."""
+    subprocess.run(["gcc", "-std=gnu99", "-O2", f"-I{REFERENCE}", f"-I{REFERENCE}/sppde", "-o", out, HDR_DRIVER_SRC,
+                    f"-L{os.path.dirname(LIB_PATH)}", "-lsw2", f"-Wl,-rpath,{os.path.dirname(LIB_PATH)}", "-lm"],
+                   check=True)
+
+
+@pytest.mark.skipif(not os.path.isdir(REFERENCE), reason="SW2's headers are not here")
+@pytest.mark.parametrize("case", [("jupiter", "jupiters"), ("saturn", "minicanal")])
+def test_caller_built_against_sw2_headers(tmp_path, golden, case):
+    """initialize, do_channel, calc_energy_balance and destroy called from an
+    object compiled against SW2's headers: the values it reads through SW2's
+    struct definitions are the library's."""
+    exe = str(tmp_path / "drv")
+    build_headers_driver(exe)
+    sub, bname = case
+    r = subprocess.run([exe, bname, str(tmp_path), "0", str(tmp_path / "x.raw")], cwd=os.path.join(golden, sub),
+                       capture_output=True, text=True, check=True)
+    tag = {l.split()[0]: l.split()[1:] for l in r.stdout.splitlines() if l.split() and l.split()[0].isupper()}
+    cwd = os.getcwd()
+    os.chdir(os.path.join(golden, sub))
+    try:
+        w = World(sw2_amd.read_case(bname + ".sw"))
+    finally:
+        os.chdir(cwd)
+    s, M = w.sw, w.sw.m
+    kv = lambda key: dict(zip(tag[key][0::2], tag[key][1::2]))
+    lay = tag["LAYOUT"]
+    assert lay[:8] == ["nx", str(s.nx), "ny", str(s.ny), "N", str(s.N), "S", str(M.S)]
+    assert " ".join(lay) == (f"nx {s.nx} ny {s.ny} N {s.N} S {M.S} sh 2 l0 1 1 l1 {s.nx} {s.ny} s {s.nx + 4} {s.ny + 4} "
+                             f"per 1 0 bc 0 0 1 1 nb 0 0 -1 -1 -1 -1")
+    v = kv("VALUES")
+    for name in ("depth", "Omega", "gE", "nu4", "Dt"):
+        assert float(v[name]) == getattr(s, name), name
+    assert float(v["sigma"]) == s.sigma_dis
+    assert int(v["gauss"]) == s.num_Gaussians and int(v["vort"]) == s.num_vortices
+    rows = tag["ROWS"]
+    assert float(rows[1]) == w.field("Dxc")[2, 2] and float(rows[2]) == w.field("Dxc")[s.ny + 1, 2]
+    assert float(rows[4]) == w.field("gc")[2, 2] and float(rows[6]) == w.field("fc")[2, 2]
+    w.do_channel()
+    ke, ape = w.energy()
+    assert [float(x) for x in tag["ENERGY0"]] == [ke, ape]
+    w.close()
+
+
+@pytest.mark.parametrize("expr,val", [("fac 20", 2432902008176640000.0), ("fac 3.7", 6.0), ("fac 0", 1.0),
+                                      ("ncr(60,30)", 118264581564861424.0), ("ncr(10,0)", 1.0), ("ncr(7,7)", 1.0),
+                                      ("npr(6,3)", 120.0)])
+def test_factorial_and_binomial_exact(expr, val):
+    """fac and ncr are exact integers wherever the value fits in 64 bits."""
+    assert sw2_amd.eval_expr(expr) == val
+
+
+def test_factorial_and_binomial_limits():
+    assert np.isinf(sw2_amd.eval_expr("fac 21")) and np.isnan(sw2_amd.eval_expr("fac(-1)"))
+    assert np.isnan(sw2_amd.eval_expr("ncr(2,3)")) and np.isinf(sw2_amd.eval_expr("ncr(1e10,2)"))
+    assert np.isinf(sw2_amd.eval_expr("ncr(68,34)"))  # 2.8e19 > 2^64
+
+
+def _case_with_line(tmp_path, old, new):
+    p = cases.scaled_params(160, 64, steps=5, lat=70.0)
+    txt, _ = cases.case_text(p, "esc", flat=True)
+    assert old in txt
+    path = tmp_path / "esc.sw"
+    path.write_text(txt.replace(old, new, 1))
+    return str(path)
+
+
+def test_expression_lines_and_escapes(tmp_path):
+    """<expr> values replaced by their %.15e value before a line is read;
+    an escaped bracket makes the rest of the line literal; a '<' without
+    '>' or a '>' without '<' is an error."""
+    q = sw2_amd.read_case(_case_with_line(tmp_path, "nx 160", "nx <2*80> \\< not an <expr> \\> here"))
+    assert q.nx == 160
+    q = sw2_amd.read_case(_case_with_line(tmp_path, "CFL 0.25", "CFL <1/3>"))
+    assert q.CFL == float("%.15e" % (1 / 3))
+    with pytest.raises(sw2_amd.SW2Error, match="missing '>'"):
+        sw2_amd.read_case(_case_with_line(tmp_path, "ny 64", "ny <64"))
+    with pytest.raises(sw2_amd.SW2Error, match="without its '<'"):
+        sw2_amd.read_case(_case_with_line(tmp_path, "ny 64", "ny 64 >"))
+    with pytest.raises(sw2_amd.SW2Error, match="can not be evaluated"):
+        sw2_amd.read_case(_case_with_line(tmp_path, "ny 64", "ny <64+>"))
+    with pytest.raises(sw2_amd.SW2Error, match="expecting an integer"):
+        sw2_amd.read_case(_case_with_line(tmp_path, "ny 64", "ny 64.5"))

@@ -113,3 +113,40 @@ def test_sw_driver_runs_saturn_case(golden, tmp_path):
     for f in FIELDS:
         a, b = w.interior(f), o.interior(f)
         assert np.array_equal(a, b), (f, np.argwhere(a != b)[:3])
+
+
+HDR_DRIVER = os.path.join(ROOT, "tests", "_bin", "sw2_headers_driver")
+
+
+@pytest.mark.skipif(not os.path.exists(HDR_DRIVER), reason="the SW2-headers caller is built only where SW2's headers are")
+def test_caller_built_against_sw2_headers_runs_loop(golden, tmp_path):
+    """An object compiled against SW2's own sw.h / sppde.h (tests/
+    sw2_headers_driver.c, built by __graft_entry__.build()) runs initialize /
+    loop / destroy through libsw2.so: after 300 steps of SW2's Jupiter case
+    its eta, u and v, read through SW2's struct definitions, are the
+    oracle's."""
+    n = 300
+    for f in os.listdir(os.path.join(golden, "jupiter")):
+        shutil.copy(os.path.join(golden, "jupiter", f), tmp_path / f)
+    raw = tmp_path / "state.raw"
+    print(f"sw2_headers_driver jupiters {n} steps", flush=True)
+    r = subprocess.run([HDR_DRIVER, "jupiters", "out", str(n), str(raw)], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=180)
+    assert r.returncode == 0, r.stderr[-2000:]
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        p = sw2_amd.read_case("jupiters.sw")
+    finally:
+        os.chdir(cwd)
+    o = Oracle(p)
+    o.preloop()
+    assert o.step(n)[0] == 0
+    S = (p.nx + 4) * (p.ny + 4)
+    got = np.fromfile(raw, dtype=np.float64).reshape(3, p.ny + 4, p.nx + 4)
+    assert got.size == 3 * S
+    for k, f in enumerate(("eta", "u", "v")):
+        assert np.array_equal(got[k], o.field(f)), f
+    ke, ape = o.energy()
+    e1 = [float(x) for l in r.stdout.splitlines() if l.startswith("ENERGY1") for x in l.split()[1:]]
+    assert e1 == [ke, ape]
