@@ -1049,15 +1049,18 @@ static int gaussian_cells(sw *SW, double lon, double lat, double amp, double sig
             r = 0;
             inj_cell *tc[8] = {NULL};
             int tn[8] = {0}, tcap[8] = {0}, terr[8] = {0};
+            /* chunk id covers rows r0..r1 whatever team the runtime grants
+             * (a smaller one under OMP_DYNAMIC, a thread limit or a caller's
+             * enclosing parallel region): the loop hands every chunk to some
+             * thread, and the lists join in chunk order below */
 #ifdef _OPENMP
-#pragma omp parallel num_threads(nt)
-            {
-                const int id = omp_get_thread_num();
+#pragma omp parallel for schedule(static, 1) num_threads(nt)
+#endif
+            for (int id = 0; id < nt; id++) {
                 const int r0 = jlo + (int)((long)nrows * id / nt), r1 = jlo + (int)((long)nrows * (id + 1) / nt) - 1;
                 terr[id] = band_cells(SW, r0, r1, lat0, Rm, amp, sigma, radius, inject, call, &tc[id], &tn[id], &tcap[id],
                                       cols, ntrust);
             }
-#endif
             int total = *n;
             for (int t = 0; t < nt; t++) {
                 if (terr[t] < 0) r = -1;

@@ -116,7 +116,6 @@ class Server:
                 waiting.popleft()
                 live[mid] = (k, ta, time.perf_counter() - t0)
                 left[mid] = (steps, nx * ny, ((nx + 63) // 64 + 1) * ((ny + 7) // 8))
-                cells_steps += float(nx) * ny * steps
             self._prefetch(waiting)
             if not live:
                 if nxt < len(reqs):
@@ -135,15 +134,18 @@ class Server:
                 st_, c_, t_ = left[mid]
                 left[mid] = (max(0, st_ - self.tick), c_, t_)
             for mid in fin:
-                left.pop(mid, None)
+                cells = left.pop(mid, (0, 0, 0))[1]
                 held.discard(mid)
                 k, ta, tadm = live.pop(mid)
                 st = self.batch.member(mid)
+                cells_steps += float(cells) * st["steps"]  # the steps it ran, not the ones it asked for
                 if on_result:
                     on_result(k, self.batch, mid)
                 done[k] = {"arrival": ta, "admitted": tadm, "completed": tnow, "steps": st["steps"],
                            "failed_step": st["failed_step"]}
                 self.batch.remove(mid)
+        for mid in live:  # runs still admitted when max_wall stopped the loop: the steps they ran
+            cells_steps += float(left[mid][1]) * self.batch.member(mid)["steps"]
         for wq in waiting:  # worlds built for requests never admitted (max_wall)
             if wq[2] is not None:
                 wq[2].result().close()

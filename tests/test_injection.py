@@ -95,3 +95,20 @@ def test_zero_volume_moment_leaves_eta_alone():
     (inc, trv, mask), (idx, sv, st, has_inc) = out[0]
     assert has_inc == 0 and np.isnan(inc.ravel()[0])
     assert np.array_equal(np.flatnonzero(mask), idx) and np.array_equal(trv[idx], st)
+
+
+@pytest.mark.parametrize("env", [{"OMP_THREAD_LIMIT": "1"}, {"OMP_DYNAMIC": "true", "OMP_THREAD_LIMIT": "2"}])
+def test_sparse_injection_with_a_smaller_thread_team(env):
+    """The band is cut into as many chunks as the host has threads (up to
+    8); when the runtime grants a smaller team (a thread limit, dynamic
+    teams, a caller's enclosing parallel region) every chunk is still
+    evaluated.  Run in a fresh process, where the OpenMP settings apply."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import sys; sys.path.insert(0, %r); "
+            "from tests.test_injection import test_sparse_injection_split_over_threads as t; t(); print('chunks ok')") % root
+    e = dict(os.environ, OMP_NUM_THREADS="8", **env)
+    r = subprocess.run([sys.executable, "-c", code], env=e, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "chunks ok" in r.stdout, r.stderr[-2000:]
