@@ -1523,6 +1523,8 @@ struct sw2_batch {
     int gsteps, blocks_eta, blocks_uv, table_dirty;
     int storms;   // members with in-loop storms: steps launched one by one, storms after each
     int *paused;  // host copy of the members' paused flags
+    int *h_tmap, *d_tmap;  // member slot of every flat tile, host and device
+    int tmap_cap;          // entries of both
 };
 
 extern "C" size_t sw2_batch_footprint(const sw *SW, int form) {
@@ -1692,15 +1694,17 @@ extern "C" int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id) {
 static void batch_launch_step(sw2_batch *b) {
     const int tiles = b->h_hdr.total_tiles > 0 ? b->h_hdr.total_tiles : 1;
     const int ge = b->blocks_eta ? b->blocks_eta : tiles, gu = b->blocks_uv ? b->blocks_uv : tiles;
+#define BATCH_PHASE(S, HV)                                                                                \
+    do {                                                                                                  \
+        if (b->blocks_eta) kbp_eta<S, HV><<<ge, NTE, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);            \
+        else kb_eta<S, HV><<<ge, NTE, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);                           \
+        if (b->blocks_uv) kbp_uv<S, HV><<<gu, NTU, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);              \
+        else kb_uv<S, HV><<<gu, NTU, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);                            \
+    } while (0)
 #define BATCH_LAUNCH(S)                                                                                   \
     case S:                                                                                               \
-        if (b->hvmode) {                                                                                  \
-            kb_eta<S, 2><<<ge, NTE, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);                            \
-            kb_uv<S, 2><<<gu, NTU, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);                             \
-        } else {                                                                                          \
-            kb_eta<S, 0><<<ge, NTE, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);                            \
-            kb_uv<S, 0><<<gu, NTU, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);                             \
-        }                                                                                                 \
+        if (b->hvmode) BATCH_PHASE(S, 2);                                                                 \
+        else BATCH_PHASE(S, 0);                                                                           \
         break;
     switch (b->sch) {
         BATCH_LAUNCH(0)
@@ -1716,11 +1720,35 @@ static void batch_launch_step(sw2_batch *b) {
             BATCH_LAUNCH(9)
     }
 #undef BATCH_LAUNCH
+#undef BATCH_PHASE
     kb_adv<<<(b->cap + 3) / 4, 256, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);
 }
 
 static int batch_upload(sw2_batch *b) {
     if (!b->table_dirty) return 0;
+    // the tile map: flat tile -> member slot, in the numbering batch_retile
+    // made (held members and empty slots own no tiles).  It grows by
+    // doubling; the kernels read its address from the header.
+    const int t = b->h_hdr.total_tiles;
+    if (t > b->tmap_cap) {
+        const int cap = t > 2 * b->tmap_cap ? t : 2 * b->tmap_cap;
+        int *nh = (int *)realloc(b->h_tmap, sizeof(int) * (size_t)cap);
+        if (!nh) {
+            sw2_set_error("out of host memory for the tile map of %d tiles", cap);
+            return -1;
+        }
+        b->h_tmap = nh;
+        HIPCHECK(hipStreamSynchronize(b->st));
+        if (b->d_tmap) HIPCHECK(hipFree(b->d_tmap));
+        b->d_tmap = NULL;
+        HIPCHECK(hipMalloc((void **)&b->d_tmap, sizeof(int) * (size_t)cap));
+        b->tmap_cap = cap;
+    }
+    for (int m = 0, k = 0; m < b->cap; m++)
+        if (!b->paused[m])
+            for (int e = b->h_mem[m].ntx * b->h_mem[m].nty; e > 0; e--) b->h_tmap[k++] = m;
+    b->h_hdr.tmap = b->d_tmap;
+    if (t > 0) HIPCHECK(hipMemcpyAsync(b->d_tmap, b->h_tmap, sizeof(int) * (size_t)t, hipMemcpyHostToDevice, b->st));
     HIPCHECK(hipMemcpyAsync(b->d_mem, b->h_mem, sizeof(BMember) * b->cap, hipMemcpyHostToDevice, b->st));
     HIPCHECK(hipMemcpyAsync(b->d_hdr, &b->h_hdr, sizeof(BHdr), hipMemcpyHostToDevice, b->st));
     HIPCHECK(hipStreamSynchronize(b->st));
@@ -1931,6 +1959,8 @@ extern "C" void sw2_batch_destroy(sw2_batch *b) {
     if (b->d_mem) (void)hipFree(b->d_mem);
     if (b->d_st) (void)hipFree(b->d_st);
     if (b->d_hdr) (void)hipFree(b->d_hdr);
+    if (b->d_tmap) (void)hipFree(b->d_tmap);
+    free(b->h_tmap);
     if (b->st) (void)hipStreamDestroy(b->st);
     free(b->sess);
     free(b->foot);
