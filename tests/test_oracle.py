@@ -36,7 +36,8 @@ def test_energy_series_of_jupiter_case(golden):
     """347 ENERGY records of SW2's JupiterEnergyTest (20 days, 34,560 steps).
     The golden file was written by SW2 version 116 (its copy is named
     out_juno_116E.txt) and the restated code is version 0.125: the series
-    agree to 6e-4 in KE and 4e-5 in APE at every record.  Dropping a term of
+    agree to 6e-4 in KE and 4e-5 in APE at every record, as SW2 0.125's own
+    run of the case does (checked below through its final state).  Dropping a term of
     the step moves them by 1.5e-3 (geofactor) to 1.8e-2 (Coriolis
     denominator), so the bound pins the structure of the step."""
     p = jupiter_params(golden)
@@ -53,6 +54,19 @@ def test_energy_series_of_jupiter_case(golden):
     ke = np.abs(got[1:, 1] / ref[1:, 1] - 1)
     ape = np.abs(got[:, 2] / ref[:, 2] - 1)
     assert ke.max() < 1e-3 and ape.max() < 1e-4, (ke.max(), ape.max())
+    # SW2 itself, built from its sources (oracle/_ref), run over the same
+    # 34,560 steps: its final state is the oracle's bit for bit, and its own
+    # ENERGY lines differ from enou.txt as the oracle's do (KE 6.813751e+20
+    # against 6.812969e+20 at t = 1.725e6 s) -- the golden file comes from
+    # an earlier version of SW2 (tests/golden/digests.json,
+    # tests/golden/make_digests.py)
+    import json
+    from tests.golden.make_digests import summary
+    with open(os.path.join(golden, "digests.json")) as f:
+        dg = json.load(f)["jupiter_34560_ref"]
+    for name in ("eta", "u", "v", "tracer", "perturbs"):
+        assert summary(o.field(name))["sha256"] == dg[name]["sha256"], name
+    assert o.energy() == (dg["ke"], dg["ape"])
 
 
 def sphere(nx=48, ny=30, lat=70.0, **kw):
