@@ -19,25 +19,30 @@ Prints one JSON line:
                 algorithmic bytes per cell-update beside it; `traffic` is the
                 measured HBM bytes per launch from the profiler summary in
                 profiles/ (byte counters scaled by a device copy), if present;
-  cpu_baseline  the oracle (the CPU restatement of SW2's path) timed on a
-                bounded sample of the same workload on this host, its row
-                sweeps on the host's CPUs (at most 16 threads), with a
-                single-thread sample beside it;
+  cpu_baseline  SW2 itself (its sources built unchanged into oracle/_ref)
+                on a bounded sample of the same workload under mpiexec, one
+                rank per host CPU (at most 16, npx = ranks, npy = 1), timed
+                by SW2's own step timer; beside it ("port") the oracle, the
+                CPU restatement of SW2's path, with its row sweeps on the same
+                number of threads and on one thread;
   mixed         the same grid in the mixed-precision form (FP32 anomalies
                 about an FP64 mean, 60 algorithmic bytes per cell-update) for
                 5000 steps, its speed-up over the FP64 form and its relative
                 L2 / max errors against the FP64 run of the same steps;
   ensemble      64 members of uneven sizes (256x128 .. 2048x1024, 1-5 seeded
                 storms each) stepped 3000 steps together in the batched mixed
-                form (persistent launches, graph-replayed step, autotuned
-                launch shape): runs/hour, against the FP64 form's runs/hour
-                from its measured per-size step times;
+                form (graph-replayed step, autotuned launch shape): runs/hour,
+                against every member run alone in the FP64 form (measured),
+                and every member's relative L2 / max error of eta, u, v
+                against its FP64 run;
   served        Poisson arrivals at 4 requests/s of storm runs (512x256 ..
                 4096x2048, 500..20000 steps) through the continuous-batching
                 loop: p50 / p99 time-to-completion and runs/hour, against
                 the FP64 form serving the same stream one run at a time
-                (first come, first served, from its measured per-size step
-                times).
+                (first come, first served; every request run in the FP64
+                form, measured), and every result's relative L2 / max error
+                against its FP64 run.  The offered load exceeds what one GPU
+                sustains, so p50 / p99 grow with the number of requests.
 """
 import argparse
 import json
@@ -61,13 +66,16 @@ def parse():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=10, help="steps of the CPU baseline sample (0: skip)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the CPU baseline (0: the host's CPUs, at most 16)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="MPI ranks of SW2 and threads of the oracle in the CPU baseline (0: the host's CPUs, at most 16)")
+    ap.add_argument("--ref-steps", type=int, default=8, help="steps of SW2 itself in the CPU baseline (0: skip)")
     ap.add_argument("--kernel-steps", type=int, default=100, help="steps timed per kernel after the timed region")
     ap.add_argument("--workload", default="c1", choices=["c0", "c1"])
     ap.add_argument("--copy-calib", action="store_true", help="only copy 3 x 512 MiB device to device and exit")
     ap.add_argument("--mixed-steps", type=int, default=5000, help="steps of the mixed-form run (0: skip)")
     ap.add_argument("--ensemble-members", type=int, default=64, help="members of the batched ensemble (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=3000)
+    ap.add_argument("--no-tune", action="store_true", help="the batch's default launch shape, no autotuning")
     ap.add_argument("--served-requests", type=int, default=40, help="requests of the served stream (0: skip)")
     ap.add_argument("--storm-steps", type=int, default=500,
                     help="steps of the 4096x2048 run with a moving storm, fused and phased (0: skip)")
@@ -96,7 +104,7 @@ def copy_calib():
     print(json.dumps({"copy_calib_bytes": 3 * n}), flush=True)
 
 
-def cpu_baseline(p_fn, bumps, steps, threads):
+def cpu_baseline_port(p_fn, bumps, steps, threads):
     """The oracle on the same workload: `steps` steps with its row sweeps on
     `threads` threads (bit-equal to its serial run), and 2 steps on one
     thread beside it."""
@@ -121,6 +129,50 @@ def cpu_baseline(p_fn, bumps, steps, threads):
             "sample": f"{p.nx}x{p.ny}, {n} steps of the same workload, {threads} threads, {dt:.1f} s "
                       f"(host: {os.cpu_count()} CPUs)",
             "single_thread": {"value": cells * n1 / dt1, "cores": 1, "sample": f"{n1} steps, {dt1:.1f} s"}}
+
+
+MPIEXEC = os.environ.get("SW2_MPIEXEC", "/opt/conda/bin/mpiexec")
+
+
+def cpu_baseline_reference(p_fn, bumps, steps, ranks):
+    """SW2 itself (oracle/_ref/sw2ref: SW2's sources built unchanged, its
+    loop() driven as main/Main.c does) on the same workload: `steps` steps
+    on `ranks` MPI ranks of this host, npx = ranks, npy = 1 (longitude
+    blocks of nx/ranks columns).  The rate is the grid's cells over the mean
+    wall time per step of SW2's own step timer (time_step00 of cr_info).
+    None when the binary or mpiexec is missing."""
+    import subprocess
+    import tempfile
+    from oracle import ref
+    from sw2_amd import cases
+    if not (os.path.exists(ref.RUN_PATH) and os.path.exists(MPIEXEC)):
+        return None
+    p = p_fn(steps)
+    w, state = cases.make_world(p, bumps)
+    w.close()
+    with tempfile.TemporaryDirectory(prefix="sw2ref_") as d:
+        ref.write_case(d, "bench", p, state)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        cmd = [MPIEXEC, "-n", str(ranks), ref.RUN_PATH, str(ranks), "1", "bench.sw", "bench", d]
+        t0 = time.perf_counter()
+        r = subprocess.run(cmd, cwd=d, env=env, capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            progress(f"SW2 reference run failed ({r.returncode}): {r.stdout[-400:]} {r.stderr[-400:]}")
+            return None
+    step_s = None
+    for line in r.stdout.splitlines():
+        f = line.split()
+        if len(f) >= 10 and f[0] == "cr_info:" and f[1] == "time_step00":
+            step_s = float(f[9])  # tavg
+    if step_s is None:
+        return None
+    cells = p.nx * p.ny
+    return {"value": cells / step_s, "unit": "cell-updates/s", "cores": ranks, "kind": "reference",
+            "ranks": ranks, "npx": ranks, "npy": 1,
+            "sample": f"SW2 (its sources built unchanged) under mpiexec, {p.nx}x{p.ny}, {steps} steps, "
+                      f"{ranks} ranks (npx {ranks}, npy 1), mean {step_s:.3f} s per step from SW2's own step "
+                      f"timer, {wall:.1f} s with set-up (host: {os.cpu_count()} CPUs)"}
 
 
 def progress(msg):
@@ -223,67 +275,134 @@ def fp64_step_ms(nx, ny, steps=40):
     return ms
 
 
-def bench_ensemble(n, steps):
+def fp64_run(p, bumps, steps):
+    """One run in the FP64 form, which equals the oracle bit for bit: its
+    wall time (world upload to the last step's completion, the world built
+    beforehand), failed step and final eta, u, v interiors."""
+    from sw2_amd import FORM_FP64, Session, cases
+    w, _ = cases.make_world(p, bumps)
+    t0 = time.perf_counter()
+    s = Session(w, FORM_FP64)
+    s.begin(0, p.t0)
+    s.timed_steps(steps)  # returns once the last step has completed
+    wall = time.perf_counter() - t0
+    failed = s.failed_step()[0]
+    s.download()
+    s.close()
+    out = {f: w.interior(f).copy() for f in FIELDS}
+    w.close()
+    return wall, failed, out
+
+
+FIELDS = ("eta", "u", "v")
+
+
+def error_summary(errs):
+    """errs: [(label, {field: {"l2", "max"}})] -> the worst L2 and max per
+    field over all results, and every result's errors."""
+    worst = {f: {"l2": max(e[f]["l2"] for _, e in errs), "max": max(e[f]["max"] for _, e in errs)} for f in FIELDS}
+    return {"results": len(errs), "worst": worst,
+            "per_result": [[lab] + [round(e[f][k], 9) for f in FIELDS for k in ("l2", "max")] for lab, e in errs],
+            "per_result_columns": ["result"] + [f"{f}_{k}" for f in FIELDS for k in ("l2", "max")]}
+
+
+def bench_ensemble(n, steps, tune_launch=True):
+    """The batched ensemble, then every member run alone in the FP64 form
+    (measured): runs/hour of both and every member's relative L2 / max
+    error of eta, u, v against its FP64 run."""
     from sw2_amd import Batch, cases, tune
     specs = cases.ensemble(n, steps)
     sizes = [(p.nx, p.ny) for p, _ in specs]
-    progress(f"ensemble: tuning the launch shape for {n} members")
-    shape, rec = tune.tune(sizes)
+    if tune_launch:
+        progress(f"ensemble: tuning the launch shape for {n} members")
+        shape, rec = tune.tune(sizes)
+    else:
+        shape, rec = None, None
     progress(f"ensemble: launch {shape}; admitting {n} members")
     b = Batch(capacity=n)
-    b.set_launch(*shape)
-    worlds = []
+    if shape:
+        b.set_launch(*shape)
+    shape = b.launch()
+    worlds, ids = [], []
     t0 = time.perf_counter()
     for p, bumps in specs:
         w, _ = cases.make_world(p, bumps)
         worlds.append(w)
-        b.add(w, steps)
+        ids.append(b.add(w, steps))
     t_admit = time.perf_counter() - t0
     progress(f"ensemble: {steps} steps")
-    b.step(shape[2])  # capture the graph (counted steps)
+    warm = min(shape[2], steps // 2)
+    b.step(warm)  # capture the graph (counted steps)
     b.sync()
     t0 = time.perf_counter()
-    b.step(steps - shape[2])
+    b.step(steps - warm)
     b.sync()
     wall = time.perf_counter() - t0
     fin = b.finished()
     failed = sum(1 for m in fin if b.member(m)["failed_step"])
+    got = []
+    for mid in ids:
+        w = b.download(mid)
+        got.append({f: w.interior(f).copy() for f in FIELDS})
     b.close()
     for w in worlds:
         w.close()
-    cells_steps = sum(float(nx) * ny for nx, ny in sizes) * (steps - shape[2])
-    wall_all = wall * steps / (steps - shape[2])
-    progress("ensemble: FP64 step times per member size")
-    ms64 = {sz: fp64_step_ms(*sz) for sz in sorted(set(sizes))}
-    t64 = sum(ms64[sz] for sz in sizes) * steps / 1e3
+    cells_steps = sum(float(nx) * ny for nx, ny in sizes) * (steps - warm)
+    wall_all = wall * steps / (steps - warm)
+    progress(f"ensemble: every member alone in the FP64 form, {steps} steps")
+    t64, errs, f64_failed = 0.0, [], 0
+    for (p, bumps), res in zip(specs, got):
+        w64, fl, ref = fp64_run(p, bumps, steps)
+        t64 += w64
+        f64_failed += fl != 0
+        errs.append((f"{p.nx}x{p.ny}", {f: rel_errors(res[f], ref[f]) for f in FIELDS}))
     return {"members": n, "steps": steps, "sizes": {f"{a}x{b_}": sizes.count((a, b_)) for a, b_ in sorted(set(sizes))},
             "runs_per_hour": n / wall_all * 3600.0, "cell_updates_per_s": cells_steps / wall, "wall_s": wall_all,
-            "completed": len(fin), "failed": failed, "launch": {"blocks_eta": shape[0], "blocks_uv": shape[1],
-                                                                   "steps_per_graph": shape[2], "tuning": rec},
-            "admit_s": t_admit, "fp64_ms_per_step": {f"{a}x{b_}": v for (a, b_), v in ms64.items()},
-            "fp64_runs_per_hour": n / t64 * 3600.0, "vs_fp64_runs_per_hour": (n / wall_all) / (n / t64)}
+            "completed": len(fin), "failed": failed,
+            "launch": {"blocks_eta": shape[0], "blocks_uv": shape[1], "steps_per_graph": shape[2], "tuning": rec},
+            "admit_s": t_admit, "fp64_wall_s": t64, "fp64_failed": f64_failed,
+            "fp64_runs_per_hour": n / t64 * 3600.0, "vs_fp64_runs_per_hour": (n / wall_all) / (n / t64),
+            "fp64_note": "every member run alone in the FP64 form, one after another (measured wall time)",
+            "errors_vs_fp64": error_summary(errs)}
 
 
 def bench_served(n):
-    from sw2_amd import cases, tune
+    """The served stream, then the FP64 form serving the same stream one run
+    at a time, first come first served: every request run in the FP64 form
+    (measured wall time each), the queue replayed on the arrival times.
+    Every result's relative L2 / max error against its FP64 run."""
+    from sw2_amd import cases
     from sw2_amd.serve import Server
     reqs = cases.served_requests(n)
+    got = {}
+
+    def keep(k, batch, mid):
+        w = batch.download(mid)
+        got[k] = {f: w.interior(f).copy() for f in FIELDS}
+
     progress(f"served stream: {n} requests")
     srv = Server(capacity=48, hbm_budget=64e9, tick_steps=40)
-    st = srv.run(reqs, log=progress)
+    st = srv.run(reqs, on_result=keep, log=progress)
     srv.close()
+    cells = [float(nx) * ny for _, nx, ny, *_ in reqs]
+    offered = sum(c * r[3] for c, r in zip(cells, reqs)) / max(r[0] for r in reqs)
     st.update({"arrivals_per_s": 4.0, "capacity": 48, "hbm_budget_bytes": 64e9, "tick_steps": 40,
-               "sizes": "512x256/1024x512/2048x1024/4096x2048 at 0.4/0.3/0.2/0.1", "steps": "uniform 500..20000"})
-    # the FP64 form serving the same stream one run at a time, first come
-    # first served, from its measured step time per size
-    progress("served stream: FP64 step times per request size")
-    ms64 = {sz: fp64_step_ms(*sz) for sz in sorted({(r[1], r[2]) for r in reqs})}
-    t = 0.0
-    for ta, nx, ny, steps, *_ in sorted(reqs, key=lambda r: r[0]):
-        t = max(t, ta) + steps * ms64[(nx, ny)] / 1e3
+               "sizes": "512x256/1024x512/2048x1024/4096x2048 at 0.4/0.3/0.2/0.1", "steps": "uniform 500..20000",
+               "offered_cell_updates_per_s": offered})
+    progress("served stream: every request alone in the FP64 form")
+    t, errs, durations = 0.0, [], {}
+    for k, (ta, nx, ny, steps, bumps) in enumerate(reqs):
+        w64, fl, ref = fp64_run(cases.member_params(nx, ny, steps), bumps, steps)
+        durations[k] = w64
+        if k in got:
+            errs.append((f"{k}:{nx}x{ny}:{steps}", {f: rel_errors(got[k][f], ref[f]) for f in FIELDS}))
+    for k, r in sorted(enumerate(reqs), key=lambda kr: kr[1][0]):
+        t = max(t, r[0]) + durations[k]
     st.update({"fp64_fcfs_wall_s": t, "fp64_runs_per_hour": len(reqs) / t * 3600.0,
                "vs_fp64_runs_per_hour": st["runs_per_hour"] / (len(reqs) / t * 3600.0),
-               "fp64_ms_per_step": {f"{a}x{b_}": v for (a, b_), v in ms64.items()}})
+               "fp64_note": "every request run alone in the FP64 form (measured wall time), first come first "
+                            "served on the same arrival times",
+               "errors_vs_fp64": error_summary(errs)})
     return st
 
 
@@ -305,11 +424,26 @@ def main():
         return
     from sw2_amd import Session, cases
 
-    progress("FP64 workload")
     wl = a.workload
     total = a.warmup + a.steps + a.kernel_steps
     p_fn = lambda n: cases.config_params(wl, steps=n)
     bumps = cases.single_bump(2024)
+    # the CPU baseline first, before this process touches the GPU (SW2 runs
+    # in child processes under mpiexec)
+    threads = a.cpu_threads if a.cpu_threads > 0 else max(1, min(16, len(os.sched_getaffinity(0))))
+    cpu = None
+    if a.ref_steps > 0:
+        progress(f"CPU baseline: SW2 on {threads} MPI ranks, {a.ref_steps} steps")
+        cpu = cpu_baseline_reference(p_fn, bumps, a.ref_steps, threads)
+    if a.cpu_steps > 0:
+        progress(f"CPU baseline: {a.cpu_steps} oracle steps on {threads} threads")
+        port = cpu_baseline_port(p_fn, bumps, a.cpu_steps, threads)
+        if cpu is None:
+            cpu = port
+        else:
+            cpu["port"] = port
+            cpu["reference_over_port"] = cpu["value"] / port["value"]
+    progress("FP64 workload")
     p = p_fn(total)
     w, _ = cases.make_world(p, bumps)
     s = Session(w)
@@ -332,10 +466,6 @@ def main():
     fused = s.fused
     s.close()
     w.close()
-    threads = a.cpu_threads if a.cpu_threads > 0 else max(1, min(16, len(os.sched_getaffinity(0))))
-    if a.cpu_steps > 0:
-        progress(f"CPU baseline: {a.cpu_steps} oracle steps on {threads} threads")
-    cpu = cpu_baseline(p_fn, bumps, a.cpu_steps, threads) if a.cpu_steps > 0 else None
     names = {"c1": "4096x2048 FP64 jet + storm, lat -85..85, Dt 10 s, del-4 5.97e10, minmod",
              "c0": "360x180 FP64 jet + storm, lat -80..80, Dt 60 s, del-4 1e15, minmod"}
     if kt:
@@ -359,7 +489,7 @@ def main():
     if a.mixed_steps > 0 and wl == "c1":
         extras["mixed"] = bench_mixed(a.mixed_steps)
     if a.ensemble_members > 0:
-        extras["ensemble"] = bench_ensemble(a.ensemble_members, a.ensemble_steps)
+        extras["ensemble"] = bench_ensemble(a.ensemble_members, a.ensemble_steps, tune_launch=not a.no_tune)
     if a.served_requests > 0:
         extras["served"] = bench_served(a.served_requests)
     if a.storm_steps > 0 and wl == "c1":
