@@ -655,7 +655,7 @@ struct sw2_session {
     Grid g;
     int S, form, sch, hv, tracer_active, nstart, n, perturbs_dirty;
     int fused;      // 0: one kernel per phase; 1: the two fused kernels per step
-    int hvmode;     // fused path: 0 none, 2 del-2 and del-4 (del-6 runs phased)
+    int hvmode;     // fused path: 0 none, 2 del-2 and del-4 in LDS, 3 Laplacians from the per-phase kernels
     int cfl_ready;  // CFL maxima of the next step already reduced
     double *eta2, *u2, *v2;  // ping-pong partners of eta, u, v on the fused path
     // mixed form: FP32 anomaly state, its ping-pong partners and histories;
@@ -843,12 +843,13 @@ extern "C" int sw2_session_create(sw *SW, int form, sw2_session **out) {
         memcpy(&b, &SW->windy[q], sizeof b);
         if (b) windy_zero = 0;
     }
-    s->fused = !force_phased && tables_ok && wind_rows_ok && windy_zero && SW->num_vortices == 0 && SW->nu6 == 0 &&
-               SW->nx >= 2 * FTX && SW->ny >= 4 && !(SW->nu2 != 0 && SW->nu4 == 0);
-    s->hvmode = s->hv ? 2 : 0;
-    if (form == SW2_FORM_MIXED && !(s->fused && !s->tracer_active)) {
-        sw2_set_error("the mixed-precision form needs a case the fused step takes: no vortices, no del-6, a zero tracer "
-                      "that no storm injects into, a j-only wind, nx >= %d, ny >= 4", 2 * FTX);
+    s->fused = !force_phased && tables_ok && wind_rows_ok && windy_zero && SW->nx >= 2 * FTX && SW->ny >= 4;
+    // del-2 with del-4 forms both passes in the kernels' LDS; del-6, or del-2
+    // alone, takes the Laplacians from the per-phase kernels (mode 3)
+    s->hvmode = !s->hv ? 0 : ((SW->nu6 == 0 && !(SW->nu2 != 0 && SW->nu4 == 0)) ? 2 : 3);
+    if (form == SW2_FORM_MIXED && !(s->fused && !s->tracer_active && s->hvmode != 3 && SW->num_vortices == 0)) {
+        sw2_set_error("the mixed-precision form needs a case the fused step takes, without vortices, del-6 or del-2 "
+                      "alone: a zero tracer that no storm injects into, a j-only wind, nx >= %d, ny >= 4", 2 * FTX);
         free(tab);
         free(s);
         return -1;
@@ -1094,24 +1095,42 @@ static int host_perturbations(sw2_session *s, int n, double t, int *eta_changed)
 }
 
 template <int SCH, int HV>
-static void launch_fused(sw2_session *s, int n, int k, int slot) {
+static void launch_fused(sw2_session *s, int n, int k, int slot, int late) {
     dim3 G((s->g.nx + FTX - 1) / FTX, (s->g.ny + FTY - 1) / FTY);
     EtaArgs ea{s->eta, s->u, s->v, s->Deta[1], s->Deta[2], s->tr, s->eta2, s->Deta[0], s->Dtr[0], n, k, slot,
                s->tracer_active, s->Dt, s->depth, 2.0 * s->Dt * (s->sigma), 2.0 * s->Dt * (s->tracer_dis), s->c2, s->c4,
-               s->courant, s->rest_x, s->rest_y};
+               s->courant, s->rest_x, s->rest_y, s->le, s->lle, s->llle, s->c6};
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i], s->st);
     kf_eta<SCH, HV><<<G, NTE, 0, s->st>>>(s->g, s->fl, ea);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 1], s->st);
     UVArgs ua{s->eta2, s->u, s->v, s->Du[1], s->Du[2], s->Dv[1], s->Dv[2], s->Dtr[0], s->Dtr[1], s->Dtr[2],
-              s->u2, s->v2, s->Du[0], s->Dv[0], s->tr, n, k, slot ^ 1, s->SW->num_Gaussians > 0, s->Dt, s->sigma, s->c2, s->c4};
+              s->u2, s->v2, s->Du[0], s->Dv[0], s->tr, n, k, slot ^ 1, late, s->Dt, s->sigma, s->c2, s->c4,
+              s->Lu, s->Lv, s->LLu, s->LLv, s->LLLu, s->LLLv, s->c6};
     kf_uv<SCH, HV><<<G, NTF, 0, s->st>>>(s->g, s->fl, ua);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 2], s->st);
 }
 
 template <int SCH>
-static void launch_fused_hv(sw2_session *s, int n, int k, int slot) {
-    if (s->hvmode) launch_fused<SCH, 2>(s, n, k, slot);
-    else launch_fused<SCH, 0>(s, n, k, slot);
+static void launch_fused_hv(sw2_session *s, int n, int k, int slot, int late) {
+    if (s->hvmode == 3) launch_fused<SCH, 3>(s, n, k, slot, late);
+    else if (s->hvmode) launch_fused<SCH, 2>(s, n, k, slot, late);
+    else launch_fused<SCH, 0>(s, n, k, slot, late);
+}
+
+// a vortex of SW2's table is added in the step that ends at time t
+// (perturbs.c:411-413)
+static int vortex_fires(const sw *SW, double Dt, double t) {
+    for (int k = 0; k <= SW->num_vortices - 1; k++) {
+        const double t0v = SW->vortices[k * N_COLS_VORTICES + 0];
+        if ((t0v < t) && (t0v >= (t - Dt))) return 1;
+    }
+    return 0;
+}
+
+// clear the CFL maxima of a slot (before they are reduced again)
+__global__ void k_cfl_clear(Flags *fl, int slot) {
+    fl->cfl[2 * slot] = 0;
+    fl->cfl[2 * slot + 1] = 0;
 }
 
 static void swap_ptr(double **a, double **b) {
@@ -1132,9 +1151,28 @@ static int step_fused(sw2_session *s) {
         k_cfl<<<G, B, 0, s->st>>>(s->g, p, s->fl, slot);
         s->cfl_ready = 1;
     }
+    // mode 3: the Laplacians of this step by the per-phase kernels, from the
+    // u, v the step starts with (calc.c:324-351, core.c:234-266)
+    if (s->hvmode == 3) {
+        lape(s, s->u, s->le);
+        if (s->lle) {
+            lape(s, s->le, s->lle);
+            if (s->llle) lape(s, s->lle, s->llle);
+        }
+        lapv(s, s->u, s->v, s->Lu, s->Lv);
+        if (s->LLu) {
+            lapv(s, s->Lu, s->Lv, s->LLu, s->LLv);
+            if (s->LLLu) lapv(s, s->LLu, s->LLv, s->LLLu, s->LLLv);
+        }
+    }
+    // a vortex added after u and v (core.c:292-296) changes eta, u and v:
+    // their checks wait for it, and the next step's CFL maxima are reduced
+    // again from the new u and v
+    const int vortex = s->SW->num_vortices > 0 && vortex_fires(s->SW, s->Dt, t);
+    const int late = ((s->SW->num_Gaussians > 0 || vortex) ? LATE_TE : 0) | (vortex ? LATE_UV : 0);
     // two kernels; the tracer is advanced inside them when it can be
     // nonzero (otherwise once u or v goes non-finite)
-    SCH_SWITCH(launch_fused_hv, s, n, k, slot);
+    SCH_SWITCH(launch_fused_hv, s, n, k, slot, late);
     swap_ptr(&s->eta, &s->eta2);
     swap_ptr(&s->u, &s->u2);
     swap_ptr(&s->v, &s->v2);
@@ -1146,15 +1184,25 @@ static int step_fused(sw2_session *s) {
         k_zero_pert<<<(s->S + 255) / 256, 256, 0, s->st>>>(s->fl, n, s->S, s->pert);
         s->perturbs_dirty = 0;
     }
-    if (s->SW->num_Gaussians) {
-        // storms after u and v (core.c:280-300), their halo images, then the
-        // checks of tracer and eta that the kernels left for after them
+    if (s->SW->num_Gaussians || vortex) {
+        // storms and vortices after u and v (core.c:280-300), their halo
+        // images, then the checks that the kernels left for after them
         int eta_changed = 0;
         if (host_perturbations(s, n, t, &eta_changed)) return -1;
         launch_halo(s, s->tr, !s->tracer_active);
         if (eta_changed) launch_halo(s, s->eta);
+        // a vortex's halo_update of u and v (perturbs.c:433-434) undid the
+        // walls the momentum kernel wrote: do_channel again (core.c:303-306)
+        if (vortex) k_channel<<<(s->g.nx + 255) / 256, 256, 0, s->st>>>(s->g, s->fl, s->u, s->v);
         k_check<<<G, B, 0, s->st>>>(s->g, s->fl, s->tr, n, 2, !s->tracer_active);
         k_check<<<G, B, 0, s->st>>>(s->g, s->fl, s->eta, n, 3, 0);
+        if (vortex) {
+            k_check<<<G, B, 0, s->st>>>(s->g, s->fl, s->u, n, 4, 0);
+            k_check<<<G, B, 0, s->st>>>(s->g, s->fl, s->v, n, 5, 0);
+            Ptrs pn{s->eta, s->u, s->v, s->wx, s->wy, s->tr};
+            k_cfl_clear<<<1, 1, 0, s->st>>>(s->fl, slot ^ 1);
+            k_cfl<<<G, B, 0, s->st>>>(s->g, pn, s->fl, slot ^ 1);
+        }
     }
     s->n = n;
     s->t = t;

@@ -347,3 +347,57 @@ def test_fused_guards():
     assert s.fused and fo == 1 and fs == 1
     assert "eta" in s.failed_step()[1]
     assert_same(w, o, "fused frozen at the stats failure")
+
+
+# ------------------------------------------------------------ vortices, del-6 and del-2 alone on the fused step
+@pytest.mark.parametrize("steps", [5, 40])
+def test_fused_vortices_with_a_storm(steps):
+    """Two vortices added mid-run (one across the periodic seam, two in the
+    same step) and a tracer-injecting storm, on the fused step and on the
+    phased step: every field equal to the oracle's.  The vortex steps add
+    eta, u, v after the momentum kernel, redo the halos and the walls, check
+    all four fields after them and reduce the next step's CFL maxima from
+    the new u and v.  Five steps end on the step the first two vortices are
+    added in (halos and walls as SW2 leaves them then)."""
+    G = np.array([[0.0, 1200.0, 200.0, 30.0, 15.0, 0, 20.0, 3e11, 1, 3.0, 6.0, 1.0]])
+    V = np.array([[600.0, 358.0, -20.0, 8.0, 5.0, -60.0, 2.0],
+                  [610.0, 120.0, 25.0, 6.0, 4.0, 40.0, 1.5],
+                  [1500.0, 200.0, 10.0, 10.0, 6.0, 30.0, 2.0]])
+    p = cases.scaled_params(160, 64, steps=steps, lat=70.0, Gaussians=G, vortices=V)
+    assert int(600.0 // p.Dt) + 1 == 5
+    for form in (0, PHASED):
+        w, o, s, fs, fo = run_both(p, cases.single_bump(6), steps, form=form)
+        assert s.fused == (form == 0)
+        assert fs == fo == 0
+        assert np.abs(w.field("tracer")).max() > 0.1
+        assert_same(w, o, f"storm + vortices, {'fused' if s.fused else 'phased'}")
+
+
+def test_fused_vortex_guard():
+    """A vortex strong enough to push its fields past SW2's 1e10 check: the
+    run fails at the vortex's step on the fused step as in the oracle, on
+    the same field, and the state is the one SW2 stops with."""
+    import re
+    V = np.array([[300.0, 100.0, 20.0, 8.0, 5.0, -1e10, 2.0]])
+    p = cases.scaled_params(160, 40, steps=12, lat=70.0, vortices=V, CFL=1e12)
+    w, o, s, fs, fo = run_both(p, cases.single_bump(9), 12)
+    assert s.fused and fo > 0 and fs == fo
+    assert fo == int(300.0 // p.Dt) + 1
+    gpu_field = re.search(r"<(\w+)>", s.failed_step()[1]).group(1)
+    assert gpu_field == re.search(r"detected in (\w+)", o.failmsg()).group(1)
+    assert_same(w, o, "fused frozen at the vortex's failure")
+
+
+@pytest.mark.parametrize("hv", ["del2+del4+del6", "del6", "del2", "del2+del6"])
+def test_fused_laplacians_from_the_phase_kernels(hv):
+    """Hyperviscosity combinations the fused kernels take from the per-phase
+    Laplacian kernels (del-6, or del-2 without del-4): equal to the oracle
+    on every limiter family used here."""
+    p = cases.scaled_params(192, 48, steps=25, lat=72.0, sch=0 if hv == "del6" else 2)
+    nu = {"del2+del4+del6": (2e9, p.hypernu[1], 3e22), "del6": (0.0, 0.0, 3e22), "del2": (3e9, 0.0, 0.0),
+          "del2+del6": (2e9, 0.0, 3e22)}[hv]
+    p.hypernu[0], p.hypernu[1], p.hypernu[2] = nu
+    w, o, s, fs, fo = run_both(p, cases.single_bump(13), 25)
+    assert s.fused
+    assert fs == fo == 0
+    assert_same(w, o, f"fused {hv}")
