@@ -25,8 +25,9 @@ Prints one JSON line:
                 by SW2's own step timer; beside it ("port") the oracle, the
                 CPU restatement of SW2's path, with its row sweeps on the same
                 number of threads and on one thread;
-  mixed         the same grid in the mixed-precision form (FP32 anomalies
-                about an FP64 mean, 60 algorithmic bytes per cell-update) for
+  mixed         the same grid in the mixed-precision form (anomalies about an
+                FP64 mean held as FP32 value + FP32 residual, FP32 fluxes,
+                84 algorithmic bytes per cell-update) for
                 5000 steps, its speed-up over the FP64 form and its relative
                 L2 / max errors against the FP64 run of the same steps;
   ensemble      64 members of uneven sizes (256x128 .. 2048x1024, 1-5 seeded
@@ -57,6 +58,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 STEP_BYTES = 120       # per cell-update: eta,u,v read+written (48) + 6 histories read (48) + 3 tendencies written (24)
 UV_BYTES = 88          # momentum kernel per cell: eta,u,v + 4 histories read (56), u,v + 2 tendencies written (32)
 ETA_BYTES = 56         # height kernel per cell: eta,u,v + 2 histories read (40), eta + 1 tendency written (16)
+MIXED_BYTES = 84       # mixed form per cell-update: eta,u,v as FP32 value + residual read and written (48),
+                       # 6 FP32 histories read (24), 3 FP32 tendencies written (12)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_4096x2048_fp64.json")
 
 
@@ -211,11 +214,11 @@ def bench_mixed(steps):
         worlds[form] = w
     cells = p.nx * p.ny
     r64, r32 = res[FORM_FP64], res[FORM_MIXED]
-    gbs = 60 * cells / (r32["dev_ms"] / (steps - 2) / 1e3) / 1e9
+    gbs = MIXED_BYTES * cells / (r32["dev_ms"] / (steps - 2) / 1e3) / 1e9
     out = {"workload": f"{p.nx}x{p.ny} jet + storm (the FP64 workload's case), {steps} steps",
            "value": cells * (steps - 2) / r32["wall"], "unit": "cell-updates/s", "dtype": "f32 anomalies / f64 mean",
            "fp64_value": cells * (steps - 2) / r64["wall"], "speedup_vs_fp64": r64["wall"] / r32["wall"],
-           "bytes_per_cell": 60, "achieved_gbs": gbs, "roofline_frac": gbs / HBM_PEAK_GBS,
+           "bytes_per_cell": MIXED_BYTES, "achieved_gbs": gbs, "roofline_frac": gbs / HBM_PEAK_GBS,
            "failed_step": r32["failed"], "fp64_failed_step": r64["failed"],
            "errors_vs_fp64": {f: rel_errors(worlds[FORM_MIXED].interior(f), worlds[FORM_FP64].interior(f)) for f in ("eta", "u", "v")},
            "mass_rel_diff": abs(r32["energy"][2] - r64["energy"][2]) / abs(r64["energy"][2]),

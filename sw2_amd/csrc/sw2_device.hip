@@ -545,10 +545,10 @@ __global__ void k_zero_pert(Flags *fl, int step, int S, double *f) {
 #include "sw2_mixed.inc"
 
 namespace {
-// storm increments (eta-only storms) on the mixed form's FP32 height anomaly
-// eta' (eta = hbar(j) + eta'), rounded once from FP64, with the periodic
-// images the height kernel writes; the marks as k_inject sets them
-__global__ void k_inject_mixed(Grid g, Flags *fl, int step, int n, const int *idx, const double *inc, int has_inc, float *eta,
+// storm increments (eta-only storms) on the mixed form's height anomaly
+// eta' (eta = hbar(j) + eta'), with the periodic images the height kernel
+// writes; the marks as k_inject sets them
+__global__ void k_inject_mixed(Grid g, Flags *fl, int step, int n, const int *idx, const double *inc, int has_inc, MField eta,
                                double *pert) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if ((fl->fail_step && (fl->fail_step != step || fl->fail_kind == 1)) || k >= n) return;
@@ -556,33 +556,43 @@ __global__ void k_inject_mixed(Grid g, Flags *fl, int step, int n, const int *id
     pert[q] = 1.0;
     if (!has_inc) return;
     const int c = q % g.sx - 1, j = q / g.sx - 1;
-    mput_periodic<float, true>(g, eta, c, j, (float)((double)eta[q] + inc[k]));
+    mput_periodic<float, true>(g, eta.hi, c, j, mset_lo(eta, q, mget(eta, q) + inc[k]));
 }
 
 // check_stats_scaf on the mixed form's eta = hbar(j) + eta' (the height
 // check the momentum kernel leaves for after the storms)
-__global__ __launch_bounds__(BX *BY) void k_check_mixed_eta(Grid g, Flags *fl, const float *eta, const double *hbar, int n) {
+__global__ __launch_bounds__(BX *BY) void k_check_mixed_eta(Grid g, Flags *fl, MField eta, const double *hbar, int n) {
     const int i = 1 + blockIdx.x * BX + threadIdx.x;
     const int j = 1 + blockIdx.y * BY + threadIdx.y;
     if (fl->fail_step && fl->fail_step != n) return;
     if (i > g.nx || j > g.ny) return;
-    const double x = hbar[j + 1] + (double)eta[IX(g, i, j)];
+    const double x = hbar[j + 1] + mget(eta, IX(g, i, j));
     if ((x < -1e10) || (x > 1e10) || (i == 1 && j == 1 && x != x)) flag_bad(fl, n, 3);
 }
 
 // device reductions of SW2's energy diagnostics (energy.c:25-55) and of the
 // mass sum H*Dxc*Dyc over the interior, in FP64, for either form: per-block
 // partial sums, then one block adding them in a fixed order
+// (the mixed form: T float with the residual arrays lo_*, which hold the
+// interior cells; the reduction reads u at i-1 = 0 and v at j-1 = 0, whose
+// values are the images of (nx, j) and the wall's zero)
 template <typename T>
 __global__ __launch_bounds__(256) void k_energy(Grid g, const T *eta, const T *u, const T *v, const double *hbar,
-                                                  double depth, double *part) {
+                                                  double depth, double *part, const float *lo_eta = nullptr,
+                                                  const float *lo_u = nullptr, const float *lo_v = nullptr) {
     double ke = 0, ape = 0, mass = 0;
+    auto val = [&](const T *f, const float *lo, int i, int j) -> double {
+        const int q = IX(g, i, j);
+        if (!lo) return (double)f[q];
+        const int iw = i < 1 ? i + g.nx : i;  // the interior cell whose residual the image shares
+        return (j < 1) ? (double)f[q] : (double)f[q] + (double)lo[IX(g, iw, j)];
+    };
     for (int q = blockIdx.x * 256 + threadIdx.x; q < g.nx * g.ny; q += gridDim.x * 256) {
         const int i = 1 + q % g.nx, j = 1 + q / g.nx;
         const double wx = TB(g, T_WROW, j);
-        const double H = (double)eta[IX(g, i, j)] + (hbar ? hbar[j + 1] : 0.0) + depth;
-        const double uc = 0.5 * (((double)u[IX(g, i, j)] + wx) + ((double)u[IX(g, i - 1, j)] + wx));
-        const double vc = 0.5 * ((double)v[IX(g, i, j)] + (double)v[IX(g, i, j - 1)]);
+        const double H = val(eta, lo_eta, i, j) + (hbar ? hbar[j + 1] : 0.0) + depth;
+        const double uc = 0.5 * ((val(u, lo_u, i, j) + wx) + (val(u, lo_u, i - 1, j) + wx));
+        const double vc = 0.5 * (val(v, lo_v, i, j) + val(v, lo_v, i, j - 1));
         const double surf = TB(g, SW2_T_DXC, j) * TB(g, SW2_T_DYC, j), gc = TB(g, SW2_T_GC, j);
         ke += 0.5 * H * (uc * uc + vc * vc) * surf - 0.5 * (wx * wx) * surf;
         ape += 0.5 * gc * (H * H) * surf - 0.5 * gc * (depth * depth) * surf;
@@ -660,7 +670,8 @@ struct sw2_session {
     double *eta2, *u2, *v2;  // ping-pong partners of eta, u, v on the fused path
     // mixed form: FP32 anomaly state, its ping-pong partners and histories;
     // FP64 mean-state tables (hbar, H0, mean pressure gradient)
-    float *m_eta, *m_u, *m_v, *m_eta2, *m_u2, *m_v2, *m_De[3], *m_Du[3], *m_Dv[3];
+    MField m_eta, m_u, m_v, m_eta2, m_u2, m_v2;
+    float *m_De[3], *m_Du[3], *m_Dv[3];
     double *d_mt, *h_hbar;
     double *d_part;          // energy reduction scratch
     unsigned long long *d_cflp;  // partial CFL maxima when stepped in a batch
@@ -713,7 +724,7 @@ static int falloc(float **p, size_t n) {
     return 0;
 }
 
-// the mixed form's mean state and FP32 anomaly fields from the world
+// the mixed form's mean state and anomaly fields from the world
 static int mixed_setup(sw2_session *s, sw *SW) {
     const int R = s->g.R, nx = SW->nx;
     const size_t S = (size_t)s->S;
@@ -736,8 +747,11 @@ static int mixed_setup(sw2_session *s, sw *SW) {
     int rc = dalloc(&s->d_mt, (size_t)MT_N * R);
     if (!rc && hipMemcpy(s->d_mt, mt, sizeof(double) * MT_N * R, hipMemcpyHostToDevice) != hipSuccess) rc = -1;
     free(mt);
-    float **fs[] = {&s->m_eta, &s->m_u, &s->m_v, &s->m_eta2, &s->m_u2, &s->m_v2};
-    for (auto f : fs) rc |= falloc(f, S);
+    MField *fs[] = {&s->m_eta, &s->m_u, &s->m_v, &s->m_eta2, &s->m_u2, &s->m_v2};
+    for (auto f : fs) {
+        rc |= falloc(&f->hi, S);
+        rc |= falloc(&f->lo, S);
+    }
     if (!rc && hipMalloc((void **)&s->d_cflp, sizeof(unsigned long long) * 2 * NPART) != hipSuccess) rc = -1;
     if (!rc && hipMemset(s->d_cflp, 0, sizeof(unsigned long long) * 2 * NPART) != hipSuccess) rc = -1;
     if (!rc && SW->num_Gaussians) { /* the storms' marks, as SW2's perturbs field */
@@ -750,21 +764,24 @@ static int mixed_setup(sw2_session *s, sw *SW) {
         rc |= falloc(&s->m_Dv[k], S);
     }
     if (rc) return -1;
-    float *he = (float *)malloc(sizeof(float) * S), *hu = (float *)malloc(sizeof(float) * S), *hv = (float *)malloc(sizeof(float) * S);
-    for (int j = -1; j <= SW->ny + 2; j++)
-        for (int i = -1; i <= nx + 2; i++) {
-            int q = LP(i, j, M);
-            he[q] = (float)(SW->eta[q] - s->h_hbar[j + 1]);
-            hu[q] = (float)SW->u[q];
-            hv[q] = (float)SW->v[q];
-        }
-    if (hipMemcpy(s->m_eta, he, sizeof(float) * S, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->m_u, hu, sizeof(float) * S, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->m_v, hv, sizeof(float) * S, hipMemcpyHostToDevice) != hipSuccess)
-        rc = -1;
-    free(he);
-    free(hu);
-    free(hv);
+    // each field split into its FP32 value and residual
+    float *hh = (float *)malloc(sizeof(float) * S), *hl = (float *)malloc(sizeof(float) * S);
+    const double *src[3] = {SW->eta, SW->u, SW->v};
+    const MField dst[3] = {s->m_eta, s->m_u, s->m_v};
+    for (int f = 0; f < 3 && !rc; f++) {
+        for (int j = -1; j <= SW->ny + 2; j++)
+            for (int i = -1; i <= nx + 2; i++) {
+                int q = LP(i, j, M);
+                const double x = src[f][q] - (f == 0 ? s->h_hbar[j + 1] : 0.0);
+                hh[q] = (float)x;
+                hl[q] = (float)(x - (double)hh[q]);
+            }
+        if (hipMemcpy(dst[f].hi, hh, sizeof(float) * S, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dst[f].lo, hl, sizeof(float) * S, hipMemcpyHostToDevice) != hipSuccess)
+            rc = -1;
+    }
+    free(hh);
+    free(hl);
     if (rc) sw2_set_error("upload failed");
     return rc;
 }
@@ -996,7 +1013,7 @@ static void lape(sw2_session *s, const double *ein, double *eo) {
 // storms (and, FP64 only, vortices) of step n at time t: on stream st, into
 // the session's fields (meta: the FP32 height anomaly that receives them in
 // the mixed form)
-static int host_perturbations_on(sw2_session *s, int n, double t, int *eta_changed, hipStream_t st, float *meta) {
+static int host_perturbations_on(sw2_session *s, int n, double t, int *eta_changed, hipStream_t st, MField meta) {
     sw *SW = s->SW;
     *eta_changed = 0;
     if (SW->num_Gaussians == 0 && SW->num_vortices == 0) return 0;
@@ -1401,7 +1418,8 @@ extern "C" int sw2_session_energy(sw2_session *s, double *ke, double *ape, doubl
     const int nb = ENERGY_PARTS;
     double *out = s->d_part + 3 * nb;
     if (s->form == SW2_FORM_MIXED)
-        k_energy<float><<<nb, 256, 0, s->st>>>(s->g, s->m_eta, s->m_u, s->m_v, s->d_mt + MT_HBAR * s->g.R, s->depth, s->d_part);
+        k_energy<float><<<nb, 256, 0, s->st>>>(s->g, s->m_eta.hi, s->m_u.hi, s->m_v.hi, s->d_mt + MT_HBAR * s->g.R, s->depth,
+                                               s->d_part, s->m_eta.lo, s->m_u.lo, s->m_v.lo);
     else
         k_energy<double><<<nb, 256, 0, s->st>>>(s->g, s->eta, s->u, s->v, nullptr, s->depth, s->d_part);
     k_energy_fin<<<1, 64, 0, s->st>>>(nb, s->d_part, out);
@@ -1427,7 +1445,8 @@ extern "C" int sw2_session_energy_noalloc(sw2_session *s, double *d_out3, void *
     // every block writes its own partials and k_energy_fin writes all three
     // outputs, so nothing in the scratch has to start at zero
     if (s->form == SW2_FORM_MIXED)
-        k_energy<float><<<nb, 256, 0, st>>>(s->g, s->m_eta, s->m_u, s->m_v, s->d_mt + MT_HBAR * s->g.R, s->depth, part);
+        k_energy<float><<<nb, 256, 0, st>>>(s->g, s->m_eta.hi, s->m_u.hi, s->m_v.hi, s->d_mt + MT_HBAR * s->g.R, s->depth, part,
+                                            s->m_eta.lo, s->m_u.lo, s->m_v.lo);
     else
         k_energy<double><<<nb, 256, 0, st>>>(s->g, s->eta, s->u, s->v, nullptr, s->depth, part);
     k_energy_fin<<<1, 64, 0, st>>>(nb, part, d_out3);
@@ -1439,22 +1458,32 @@ extern "C" int sw2_session_download(sw2_session *s, sw *SW) {
     size_t S = (size_t)s->S;
     HIPCHECK(hipStreamSynchronize(s->st));
     if (s->form == SW2_FORM_MIXED) {
-        float *h = (float *)malloc(sizeof(float) * S);
-        float *src[3] = {s->m_eta, s->m_u, s->m_v};
+        float *h = (float *)malloc(sizeof(float) * S), *hl = (float *)malloc(sizeof(float) * S);
+        MField src[3] = {s->m_eta, s->m_u, s->m_v};
         double *dst[3] = {SW->eta, SW->u, SW->v};
         int rc = 0;
         for (int f = 0; f < 3 && !rc; f++) {
-            if (hipMemcpy(h, src[f], sizeof(float) * S, hipMemcpyDeviceToHost) != hipSuccess) {
+            if (hipMemcpy(h, src[f].hi, sizeof(float) * S, hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(hl, src[f].lo, sizeof(float) * S, hipMemcpyDeviceToHost) != hipSuccess) {
                 rc = -1;
                 break;
             }
+            // interior cells: value + residual; halo cells: the value of
+            // their image, completed by the residual of the interior cell
+            // it images (the walls' copies and zeros as stored)
             for (int j = -1; j <= SW->ny + 2; j++)
                 for (int i = -1; i <= SW->nx + 2; i++) {
                     int q = LP(i, j, SW->M);
-                    dst[f][q] = (double)h[q] + (f == 0 ? s->h_hbar[j + 1] : 0.0);
+                    double x = (double)h[q];
+                    if (j >= 1 && j <= SW->ny) {
+                        const int ii = i < 1 ? i + SW->nx : (i > SW->nx ? i - SW->nx : i);
+                        x += (double)hl[LP(ii, j, SW->M)];
+                    }
+                    dst[f][q] = x + (f == 0 ? s->h_hbar[j + 1] : 0.0);
                 }
         }
         free(h);
+        free(hl);
         if (!rc && s->pert && hipMemcpy(SW->perturbs, s->pert, sizeof(double) * S, hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
         if (rc) sw2_set_error("download failed");
         return rc;
@@ -1512,9 +1541,11 @@ extern "C" void sw2_session_destroy(sw2_session *s) {
         if (s->Dv[k]) (void)hipFree(s->Dv[k]);
         if (s->Dtr[k]) (void)hipFree(s->Dtr[k]);
     }
-    float *fall[] = {s->m_eta, s->m_u, s->m_v, s->m_eta2, s->m_u2, s->m_v2};
-    for (auto p : fall)
-        if (p) (void)hipFree(p);
+    MField fall[] = {s->m_eta, s->m_u, s->m_v, s->m_eta2, s->m_u2, s->m_v2};
+    for (auto p : fall) {
+        if (p.hi) (void)hipFree(p.hi);
+        if (p.lo) (void)hipFree(p.lo);
+    }
     for (int k = 0; k < 3; k++) {
         if (s->m_De[k]) (void)hipFree(s->m_De[k]);
         if (s->m_Du[k]) (void)hipFree(s->m_Du[k]);
@@ -1578,7 +1609,7 @@ struct sw2_batch {
 extern "C" size_t sw2_batch_footprint(const sw *SW, int form) {
     const size_t S = (size_t)(SW->nx + 4) * (SW->ny + 4), R = (size_t)SW->ny + 4;
     size_t f = (size_t)(SW2_NT + 2) * R * sizeof(double) + 3 * R * sizeof(double) + (3 * 1024 + 3) * sizeof(double) + sizeof(Flags);
-    if (form == SW2_FORM_MIXED) f += 15 * S * sizeof(float);
+    if (form == SW2_FORM_MIXED) f += 12 * S * sizeof(float) + 9 * S * sizeof(float);
     else f += 24 * S * sizeof(double);
     if (SW->num_Gaussians) f += S * sizeof(double); /* the storms' marks */
     return f;
@@ -1852,7 +1883,7 @@ static int batch_storms(sw2_batch *b) {
         if (!s || !M.late || b->paused[m] || s->n >= M.nlast) continue;
         const int n = s->n + 1;
         const double t = s->t + s->Dt;
-        float *eta_new = M.eta[((n - M.nstart - 1) & 1) ^ 1];
+        MField eta_new = M.eta[((n - M.nstart - 1) & 1) ^ 1];
         if (s->perturbs_dirty) {
             k_zero_pert<<<(s->S + 255) / 256, 256, 0, b->st>>>(s->fl, n, s->S, s->pert);
             s->perturbs_dirty = 0;
