@@ -324,8 +324,8 @@ def bench_ensemble(n, steps, tune_launch=True):
     progress(f"ensemble: launch {shape}; admitting {n} members")
     b = Batch(capacity=n)
     if shape:
-        b.set_launch(*shape)
-    shape = b.launch()
+        tune.apply(b, shape)
+    shape = tuple(b.launch()) + tuple(b.tiles())
     worlds, ids = [], []
     t0 = time.perf_counter()
     for p, bumps in specs:
@@ -362,7 +362,8 @@ def bench_ensemble(n, steps, tune_launch=True):
     return {"members": n, "steps": steps, "sizes": {f"{a}x{b_}": sizes.count((a, b_)) for a, b_ in sorted(set(sizes))},
             "runs_per_hour": n / wall_all * 3600.0, "cell_updates_per_s": cells_steps / wall, "wall_s": wall_all,
             "completed": len(fin), "failed": failed,
-            "launch": {"blocks_eta": shape[0], "blocks_uv": shape[1], "steps_per_graph": shape[2], "tuning": rec},
+            "launch": {"blocks_eta": shape[0], "blocks_uv": shape[1], "steps_per_graph": shape[2],
+                       "tiles": [shape[3], shape[4]], "tuning": rec},
             "admit_s": t_admit, "fp64_wall_s": t64, "fp64_failed": f64_failed,
             "fp64_runs_per_hour": n / t64 * 3600.0, "vs_fp64_runs_per_hour": (n / wall_all) / (n / t64),
             "fp64_note": "every member run alone in the FP64 form, one after another (measured wall time)",
@@ -383,15 +384,19 @@ def bench_served(n):
         w = batch.download(mid)
         got[k] = {f: w.interior(f).copy() for f in FIELDS}
 
-    progress(f"served stream: {n} requests")
-    srv = Server(capacity=48, hbm_budget=64e9, tick_steps=40)
+    # the launch and tile shape tuned on a batch of the stream's size mix
+    from sw2_amd import tune
+    mix = [(512, 256)] * 4 + [(1024, 512)] * 3 + [(2048, 1024)] * 2 + [(4096, 2048)]
+    shape, rec = tune.tune(mix)
+    progress(f"served stream: {n} requests, launch {shape}")
+    srv = Server(capacity=48, hbm_budget=64e9, tick_steps=40, launch=shape)
     st = srv.run(reqs, on_result=keep, log=progress)
     srv.close()
     cells = [float(nx) * ny for _, nx, ny, *_ in reqs]
     offered = sum(c * r[3] for c, r in zip(cells, reqs)) / max(r[0] for r in reqs)
     st.update({"arrivals_per_s": 4.0, "capacity": 48, "hbm_budget_bytes": 64e9, "tick_steps": 40,
                "sizes": "512x256/1024x512/2048x1024/4096x2048 at 0.4/0.3/0.2/0.1", "steps": "uniform 500..20000",
-               "offered_cell_updates_per_s": offered})
+               "offered_cell_updates_per_s": offered, "launch": list(shape), "tuning": rec})
     progress("served stream: every request alone in the FP64 form")
     t, errs, durations = 0.0, [], {}
     for k, (ta, nx, ny, steps, bumps) in enumerate(reqs):

@@ -294,8 +294,8 @@ int sw2_session_failed_step(sw2_session *s, char *msg, int msglen);
 /* Dt_CFL of the most recent step (compute_Dt, calc.c:701-738). */
 int sw2_session_last_cfl(sw2_session *s, double *dtcfl);
 /* 1 when the session steps with the two fused kernels, 0 when it runs one
- * kernel per phase (in-loop storms or vortices, del-6, active tracer, grids
- * narrower than 128 columns, or SW2_PHASED). */
+ * kernel per phase (grids narrower than 128 columns, a wind table that is
+ * not one value per row, or SW2_PHASED). */
 int sw2_session_fused(sw2_session *s);
 /* SW2_FORM_FP64 or SW2_FORM_MIXED. */
 int sw2_session_form(sw2_session *s);
@@ -306,10 +306,12 @@ void sw2_session_destroy(sw2_session *s);
 /* ---------------------------------------------------------------------------
  * Batches (no counterpart in SW2): many worlds of any sizes stepped together
  * in the mixed-precision form.  Every active member's tiles go through one
- * persistent launch per phase; the step is captured once in a hipGraph and
- * replayed; members join and leave between steps (continuous batching)
- * without recapture.  Members must be cases the fused step takes and share
- * the limiter and hyperviscosity order.
+ * launch per phase (by default one workgroup per tile; persistent
+ * workgroups as a set launch shape); the steps are captured in a hipGraph
+ * and replayed, captured again when the tile count changes (members joining,
+ * leaving, held or released) while launching one workgroup per tile, and
+ * launched one by one while a member has in-loop storms.  Members must be
+ * cases the mixed form takes and share the limiter and hyperviscosity order.
  * ------------------------------------------------------------------------- */
 typedef struct sw2_batch sw2_batch;
 
@@ -331,6 +333,11 @@ int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id);
  * batch starts with one workgroup per tile and 8 steps per graph. */
 int sw2_batch_set_launch(sw2_batch *b, int blocks_eta, int blocks_uv, int steps_per_graph);
 int sw2_batch_get_launch(sw2_batch *b, int *blocks_eta, int *blocks_uv, int *steps_per_graph);
+/* Tile shape of the batch kernels, columns x rows: 32 x 16 (a new batch's),
+ * 64 x 8 or 64 x 16 (one workgroup per tile only); any time between steps,
+ * members included.  Results do not depend on the shape. */
+int sw2_batch_set_tiles(sw2_batch *b, int tile_x, int tile_y);
+int sw2_batch_get_tiles(sw2_batch *b, int *tile_x, int *tile_y);
 /* Queue nsteps steps of every active member (each stops at its own last
  * step); returns without waiting. */
 int sw2_batch_step(sw2_batch *b, int nsteps);

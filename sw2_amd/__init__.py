@@ -250,6 +250,15 @@ class Batch:
         lib().sw2_batch_get_launch(self.h, byref(a), byref(b), byref(c))
         return a.value, b.value, c.value
 
+    def set_tiles(self, tile_x, tile_y):
+        """Tile shape of the batch kernels: (32, 16), the default, (64, 8) or (64, 16)."""
+        _check(lib().sw2_batch_set_tiles(self.h, tile_x, tile_y), "sw2_batch_set_tiles")
+
+    def tiles(self):
+        a, b = c_int(), c_int()
+        lib().sw2_batch_get_tiles(self.h, byref(a), byref(b))
+        return a.value, b.value
+
     def step(self, n):
         """Queue n steps of every active member (returns without waiting)."""
         _check(lib().sw2_batch_step(self.h, n), "sw2_batch_step")

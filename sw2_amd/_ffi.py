@@ -91,6 +91,7 @@ EXPORTS = [
     "sw2_session_energy", "sw2_session_energy_noalloc", "sw2_session_download", "sw2_session_form",
     "sw2_session_failed_step", "sw2_session_last_cfl", "sw2_session_fused", "sw2_session_state", "sw2_session_destroy",
     "sw2_device_cus", "sw2_batch_footprint", "sw2_batch_create", "sw2_batch_add", "sw2_batch_set_launch", "sw2_batch_get_launch",
+    "sw2_batch_set_tiles", "sw2_batch_get_tiles",
     "sw2_batch_step", "sw2_batch_sync", "sw2_batch_pause", "sw2_batch_member", "sw2_batch_finished", "sw2_batch_download",
     "sw2_batch_energy", "sw2_batch_remove", "sw2_batch_count", "sw2_batch_destroy",
     "sw2_last_error", "sw2_build_info",
@@ -158,6 +159,8 @@ def lib():
     L.sw2_batch_add.argtypes = [c_void_p, POINTER(SW), c_int, PI]
     L.sw2_batch_set_launch.argtypes = [c_void_p, c_int, c_int, c_int]
     L.sw2_batch_get_launch.argtypes = [c_void_p, PI, PI, PI]
+    L.sw2_batch_set_tiles.argtypes = [c_void_p, c_int, c_int]
+    L.sw2_batch_get_tiles.argtypes = [c_void_p, PI, PI]
     L.sw2_batch_step.argtypes = [c_void_p, c_int]
     L.sw2_batch_sync.argtypes = [c_void_p]
     L.sw2_batch_pause.argtypes = [c_void_p, c_int, c_int]

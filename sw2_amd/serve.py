@@ -47,8 +47,12 @@ class Server:
         if policy not in ("srpt", "all"):
             raise ValueError(f"policy {policy!r}: 'srpt' or 'all'")
         self.batch = Batch(capacity=capacity, hbm_budget=hbm_budget)
-        if launch:
-            self.batch.set_launch(*launch)
+        if launch:  # (blocks_eta, blocks_uv, steps_per_graph[, tile_x, tile_y]), as the autotuner gives it
+            from . import tune
+            if len(launch) == 5:
+                tune.apply(self.batch, launch)
+            else:
+                self.batch.set_launch(*launch)
         self.tick = tick_steps
         self.pool = ThreadPoolExecutor(max_workers=builders)
         self.prefetch = prefetch

@@ -1,14 +1,18 @@
-"""Autotuner of the batched step's launch shape.
+"""Autotuner of the batched step's launch shape and tile shape.
 
 For a mix of member grid sizes it builds a scratch batch of the same
 composition (members at rest: the mixed-form kernels take the same path
-whatever the data), times whole steps for candidate launch shapes and keeps
-the fastest.  The knobs are the persistent workgroups of the height kernel
-and of the momentum kernel (multiples of the device's compute-unit count)
-and the steps per captured graph; the search is coordinate descent from the
-batch's default (the workgroups that stay resident at once).  Results are
-cached per device and mix in a JSON file (default: tuned_launch.json beside
-this module, or $SW2_TUNE_CACHE).
+whatever the data), times whole steps for candidate shapes and keeps the
+fastest.  The knobs are the tile shape of the batch kernels (32 x 16,
+64 x 8 or 64 x 16 cells), the persistent workgroups of the height kernel
+and of the momentum kernel (multiples of the device's compute-unit count;
+32 x 16 tiles only) and the steps per captured graph.  The search is
+coordinate descent from the batch's default (one workgroup per 32 x 16
+tile, 8 steps per graph): tile shapes first, then the workgroup counts,
+then the steps per graph.  A shape is (blocks_eta, blocks_uv,
+steps_per_graph, tile_x, tile_y).  Results are cached per device and mix in
+a JSON file (default: tuned_launch.json beside this module, or
+$SW2_TUNE_CACHE).
 """
 import json
 import os
@@ -31,8 +35,19 @@ def mix_key(sizes, ncu):
     return f"cu{ncu}|mixed|" + ",".join(f"{nx}x{ny}:{k}" for (nx, ny), k in sorted(c.items()))
 
 
+TILE_SHAPES = [(32, 16), (64, 8), (64, 16)]  # the first is a new batch's
+
+
+def apply(b, shape):
+    """Set a batch to a shape (blocks_eta, blocks_uv, steps_per_graph,
+    tile_x, tile_y)."""
+    b.set_launch(0, 0, shape[2])  # persistent workgroups exist for the default tiles only
+    b.set_tiles(shape[3], shape[4])
+    b.set_launch(*shape[:3])
+
+
 def _time_steps(b, shape, steps):
-    b.set_launch(*shape)
+    apply(b, shape)
     b.step(shape[2])  # capture and warm up
     b.sync()
     t0 = time.perf_counter()
@@ -42,8 +57,8 @@ def _time_steps(b, shape, steps):
 
 
 def tune(sizes, steps=16, cache=CACHE, verbose=False):
-    """Best (blocks_eta, blocks_uv, steps_per_graph) for members of `sizes`
-    ([(nx, ny), ...]); returns (shape, record)."""
+    """Best (blocks_eta, blocks_uv, steps_per_graph, tile_x, tile_y) for
+    members of `sizes` ([(nx, ny), ...]); returns (shape, record)."""
     ncu = device_cus()
     key = mix_key(sizes, ncu)
     db = {}
@@ -53,7 +68,7 @@ def tune(sizes, steps=16, cache=CACHE, verbose=False):
                 db = json.load(f)
         except (OSError, ValueError):
             db = {}
-    if key in db:
+    if key in db and len(db[key]["shape"]) == 5:
         r = db[key]
         return tuple(r["shape"]), r
     b = Batch(capacity=len(sizes))
@@ -63,15 +78,21 @@ def tune(sizes, steps=16, cache=CACHE, verbose=False):
         worlds.append(w)
         b.add(w, 1 << 30)
     t_start = time.perf_counter()
-    default = b.launch()  # one workgroup per tile, 8 steps per graph
+    default = tuple(b.launch()) + tuple(b.tiles())  # one workgroup per 64 x 8 tile, 8 steps per graph
     best = default
     best_t = _time_steps(b, best, steps)
     tried = {best: best_t}
-    for axis, options in ((0, [1, 2, 3, 4, 6]), (1, [1, 2, 3, 4, 6]), (2, [4, 8, 16])):
+    axes = [(3, TILE_SHAPES), (0, [1, 2, 3, 4, 6]), (1, [1, 2, 3, 4, 6]), (2, [4, 8, 16])]
+    for axis, options in axes:
         for o in options:
             shape = list(best)
-            shape[axis] = o * ncu if axis < 2 else o
+            if axis == 3:
+                shape[3:5] = o
+            else:
+                shape[axis] = o * ncu if axis < 2 else o
             shape = tuple(shape)
+            if (shape[0] or shape[1]) and shape[3:5] != TILE_SHAPES[0]:
+                continue  # persistent workgroups run the default tiles only
             if shape in tried:
                 continue
             t = _time_steps(b, shape, steps)

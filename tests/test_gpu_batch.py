@@ -187,3 +187,41 @@ def test_all_members_held_then_released():
         assert st["done"] and st["steps"] == steps and st["failed_step"] == 0, st
         same(b.download(mid), solo(p, bumps, steps), f"member {mid}")
     b.close()
+
+
+@pytest.mark.parametrize("tiles", [(64, 8), (64, 16)])
+def test_tile_shapes_equal_solo_runs(tiles):
+    """The batch kernels on taller tiles: every member (ragged sizes, storms,
+    one held for a while, the shape switched mid-run) equals its solo run
+    bit for bit -- a cell's arithmetic does not depend on the tile that
+    computes it."""
+    members = small_members(7, 6)
+    G = np.array([[0.0, 1e9, 100.0, 40.0, 20.0, 0, 30.0, 2e11, 1, 3.0, 8.0, 0.0]])
+    p_storm = cases.member_params(256, 128, 50, Gaussians=G)
+    members.append((p_storm, cases.storm_bumps(np.random.default_rng(3)), 50))
+    b = Batch(capacity=8)
+    default = b.tiles()
+    assert default == (32, 16)
+    b.set_tiles(*tiles)
+    assert b.tiles() == tiles
+    ids = []
+    for p, bumps, steps in members:
+        w, _ = cases.make_world(p, bumps)
+        ids.append(b.add(w, steps))
+    b.pause(ids[1])
+    b.step(10)
+    b.pause(ids[1], False)
+    b.set_tiles(*default)  # between steps, members included
+    b.step(7)
+    b.set_tiles(*tiles)
+    b.step(max(m[2] for m in members) + 5)
+    b.sync()
+    for mid, (p, bumps, steps) in zip(ids, members):
+        st = b.member(mid)
+        assert st["done"] and st["steps"] == steps and st["failed_step"] == 0, st
+        same(b.download(mid), solo(p, bumps, steps), f"member {mid} on {tiles} tiles")
+    with pytest.raises(SW2Error, match="persistent workgroups"):
+        b.set_launch(256, 0, 8)
+    with pytest.raises(SW2Error, match="no batch kernels"):
+        b.set_tiles(48, 8)
+    b.close()
