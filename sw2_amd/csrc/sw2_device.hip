@@ -1711,6 +1711,8 @@ extern "C" int sw2_batch_create(int form, int capacity, double hbm_budget_bytes,
     }
     b->h_hdr.nslots = capacity;
     b->h_hdr.total_tiles = 0;
+    b->h_hdr.xblock = 256;  // tiles per XCD block: 64..1024 measured within 2% of each other on 32 x 16 tiles
+    if (const char *e = getenv("SW2_XCD_BLOCK")) b->h_hdr.xblock = atoi(e) > 0 ? atoi(e) : 256;
     b->table_dirty = 1;
     if (rc) {
         sw2_set_error("batch allocation failed");

@@ -46,17 +46,22 @@ def apply(b, shape):
     b.set_launch(*shape[:3])
 
 
-def _time_steps(b, shape, steps):
+def _time_steps(b, shape, steps, repeats=3):
+    """Seconds per step of a shape: the fastest of `repeats` timings of
+    `steps` steps (a single short timing picked shapes by noise)."""
     apply(b, shape)
     b.step(shape[2])  # capture and warm up
     b.sync()
-    t0 = time.perf_counter()
-    b.step(steps)
-    b.sync()
-    return (time.perf_counter() - t0) / steps
+    best = float("inf")
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.sync()
+        best = min(best, (time.perf_counter() - t0) / steps)
+    return best
 
 
-def tune(sizes, steps=16, cache=CACHE, verbose=False):
+def tune(sizes, steps=32, cache=CACHE, verbose=False):
     """Best (blocks_eta, blocks_uv, steps_per_graph, tile_x, tile_y) for
     members of `sizes` ([(nx, ny), ...]); returns (shape, record)."""
     ncu = device_cus()
