@@ -241,10 +241,12 @@ int load_binary_sw(int frame, int *nstep, double *timedouble, char *bname, char 
 
 /* Forms of the device step. */
 #define SW2_FORM_FP64  0   /* plain FP64, same results as SW2's CPU path */
-#define SW2_FORM_MIXED 1   /* FP32 anomalies about an FP64 mean state: FP32 fluxes,
-                              * limiters and Laplacians; FP64 mean terms, Adams-Bashforth
+#define SW2_FORM_MIXED 1   /* anomalies about an FP64 mean state (FP32 value + FP32
+                              * residual per cell): FP32 fluxes, limiters, pressure
+                              * gradients and Laplacians; FP64 mean terms, Adams-Bashforth
                               * accumulation, Coriolis update and reductions.  Cases the
-                              * fused step takes only (sw2_session_fused). */
+                              * fused step takes (sw2_session_fused) without vortices,
+                              * del-6 or del-2 alone; storms with or without tracer. */
 /* Flag or'ed into the form: run one kernel per phase of the step even where
  * the fused two-kernel step applies (same results; for cross-checks). */
 #define SW2_PHASED 0x100
@@ -287,7 +289,8 @@ int sw2_session_energy(sw2_session *s, double *ke, double *ape, double *mass);
  * session's device current. */
 int sw2_session_energy_noalloc(sw2_session *s, double *d_out3, void *stream);
 /* Copy eta, u, v, tracer, perturbs (halos included) back into SW's arrays.
- * A mixed-form session writes its FP64 mean plus FP32 anomaly. */
+ * A mixed-form session writes its FP64 mean plus its anomaly (value +
+ * residual). */
 int sw2_session_download(sw2_session *s, sw *SW);
 /* First step whose CFL or stats guard failed (0 if none) and its message. */
 int sw2_session_failed_step(sw2_session *s, char *msg, int msglen);

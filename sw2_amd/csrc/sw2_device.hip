@@ -585,29 +585,31 @@ namespace tile_64x16 {
 #undef SW2_TILE_Y
 
 namespace {
-// storm increments (eta-only storms) on the mixed form's height anomaly
-// eta' (eta = hbar(j) + eta'), with the periodic images the height kernel
-// writes; the marks as k_inject sets them
-__global__ void k_inject_mixed(Grid g, Flags *fl, int step, int n, const int *idx, const double *inc, int has_inc, MField eta,
-                               double *pert) {
+// storm injections on the mixed form (perturbs.c:348-395): the marks as
+// k_inject sets them, the eta increment on the height anomaly eta' (eta =
+// hbar(j) + eta') and, for storms that inject tracer, the tracer value
+// (tr_on), each written with its periodic images as the kernels write them
+__global__ void k_inject_mixed(Grid g, Flags *fl, int step, int n, const int *idx, const double *inc, const double *trv,
+                               int has_inc, MField eta, MField tr, int tr_on, double *pert) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if ((fl->fail_step && (fl->fail_step != step || fl->fail_kind == 1)) || k >= n) return;
     const int q = idx[k];
     pert[q] = 1.0;
-    if (!has_inc) return;
     const int c = q % g.sx - 1, j = q / g.sx - 1;
+    if (tr_on) mput_periodic<float, true>(g, tr.hi, c, j, mset_lo(tr, q, trv[k]));
+    if (!has_inc) return;
     mput_periodic<float, true>(g, eta.hi, c, j, mset_lo(eta, q, mget(eta, q) + inc[k]));
 }
 
-// check_stats_scaf on the mixed form's eta = hbar(j) + eta' (the height
-// check the momentum kernel leaves for after the storms)
-__global__ __launch_bounds__(BX *BY) void k_check_mixed_eta(Grid g, Flags *fl, MField eta, const double *hbar, int n) {
+// check_stats_scaf on a mixed form field (hbar(j) + the anomaly, for eta):
+// the checks the momentum kernel leaves for after the storms
+__global__ __launch_bounds__(BX *BY) void k_check_mixed(Grid g, Flags *fl, MField f, const double *hbar, int n, int kind) {
     const int i = 1 + blockIdx.x * BX + threadIdx.x;
     const int j = 1 + blockIdx.y * BY + threadIdx.y;
     if (fl->fail_step && fl->fail_step != n) return;
     if (i > g.nx || j > g.ny) return;
-    const double x = hbar[j + 1] + mget(eta, IX(g, i, j));
-    if ((x < -1e10) || (x > 1e10) || (i == 1 && j == 1 && x != x)) flag_bad(fl, n, 3);
+    const double x = (hbar ? hbar[j + 1] : 0.0) + mget(f, IX(g, i, j));
+    if ((x < -1e10) || (x > 1e10) || (i == 1 && j == 1 && x != x)) flag_bad(fl, n, kind);
 }
 
 // device reductions of SW2's energy diagnostics (energy.c:25-55) and of the
@@ -712,6 +714,8 @@ struct sw2_session {
     // FP64 mean-state tables (hbar, H0, mean pressure gradient)
     MField m_eta, m_u, m_v, m_eta2, m_u2, m_v2;
     float *m_De[3], *m_Du[3], *m_Dv[3];
+    MField m_tr;      // the tracer (tracer_active), updated in place
+    float *m_Dtr[3];  // and its tendencies
     double *d_mt, *h_hbar;
     double *d_part;          // energy reduction scratch
     unsigned long long *d_cflp;  // partial CFL maxima when stepped in a batch
@@ -803,12 +807,17 @@ static int mixed_setup(sw2_session *s, sw *SW) {
         rc |= falloc(&s->m_Du[k], S);
         rc |= falloc(&s->m_Dv[k], S);
     }
+    if (s->tracer_active) {
+        rc |= falloc(&s->m_tr.hi, S);
+        rc |= falloc(&s->m_tr.lo, S);
+        for (int k = 0; k < 3; k++) rc |= falloc(&s->m_Dtr[k], S);
+    }
     if (rc) return -1;
     // each field split into its FP32 value and residual
     float *hh = (float *)malloc(sizeof(float) * S), *hl = (float *)malloc(sizeof(float) * S);
-    const double *src[3] = {SW->eta, SW->u, SW->v};
-    const MField dst[3] = {s->m_eta, s->m_u, s->m_v};
-    for (int f = 0; f < 3 && !rc; f++) {
+    const double *src[4] = {SW->eta, SW->u, SW->v, SW->tracer};
+    const MField dst[4] = {s->m_eta, s->m_u, s->m_v, s->m_tr};
+    for (int f = 0; f < (s->tracer_active ? 4 : 3) && !rc; f++) {
         for (int j = -1; j <= SW->ny + 2; j++)
             for (int i = -1; i <= nx + 2; i++) {
                 int q = LP(i, j, M);
@@ -904,9 +913,9 @@ extern "C" int sw2_session_create(sw *SW, int form, sw2_session **out) {
     // del-2 with del-4 forms both passes in the kernels' LDS; del-6, or del-2
     // alone, takes the Laplacians from the per-phase kernels (mode 3)
     s->hvmode = !s->hv ? 0 : ((SW->nu6 == 0 && !(SW->nu2 != 0 && SW->nu4 == 0)) ? 2 : 3);
-    if (form == SW2_FORM_MIXED && !(s->fused && !s->tracer_active && s->hvmode != 3 && SW->num_vortices == 0)) {
+    if (form == SW2_FORM_MIXED && !(s->fused && s->hvmode != 3 && SW->num_vortices == 0)) {
         sw2_set_error("the mixed-precision form needs a case the fused step takes, without vortices, del-6 or del-2 "
-                      "alone: a zero tracer that no storm injects into, a j-only wind, nx >= 128, ny >= 4");
+                      "alone: a j-only wind, nx >= 128, ny >= 4");
         free(tab);
         free(s);
         return -1;
@@ -1111,8 +1120,8 @@ static int host_perturbations_on(sw2_session *s, int n, double t, int *eta_chang
             HIPCHECK(hipMemcpyAsync(s->d_inc[k], h_inc, sizeof(double) * (size_t)x.n, hipMemcpyHostToDevice, st));
             HIPCHECK(hipMemcpyAsync(s->d_trv[k], h_trv, sizeof(double) * (size_t)x.n, hipMemcpyHostToDevice, st));
             if (s->form == SW2_FORM_MIXED)
-                k_inject_mixed<<<(x.n + 255) / 256, 256, 0, st>>>(s->g, s->fl, n, x.n, s->d_idx[k], s->d_inc[k], x.has_inc,
-                                                                    meta, s->pert);
+                k_inject_mixed<<<(x.n + 255) / 256, 256, 0, st>>>(s->g, s->fl, n, x.n, s->d_idx[k], s->d_inc[k], s->d_trv[k],
+                                                                    x.has_inc, meta, s->m_tr, s->tracer_active, s->pert);
             else
                 k_inject<<<(x.n + 255) / 256, 256, 0, st>>>(s->fl, n, x.n, s->d_idx[k], s->d_inc[k], s->d_trv[k], x.has_inc,
                                                               s->eta, s->tr, s->pert);
@@ -1272,12 +1281,14 @@ template <int SCH, int HV>
 static void launch_mixed(sw2_session *s, int n, int k, int slot) {
     dim3 G((s->g.nx + FTX - 1) / FTX, (s->g.ny + FTY - 1) / FTY);
     MEtaArgs ea{s->m_eta, s->m_u, s->m_v, s->m_De[1], s->m_De[2], s->m_eta2, s->m_De[0], s->d_mt, n, k, slot,
-                s->Dt, 2.0 * s->Dt * (s->sigma), s->c2, s->c4, s->courant, s->rest_x, s->rest_y};
+                s->Dt, 2.0 * s->Dt * (s->sigma), s->c2, s->c4, s->courant, s->rest_x, s->rest_y,
+                s->m_tr, s->m_Dtr[0], s->tracer_active, 2.0 * s->Dt * (s->tracer_dis)};
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i], s->st);
     km_eta<SCH, HV><<<G, NTE, 0, s->st>>>(s->g, s->fl, ea);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 1], s->st);
     MUVArgs ua{s->m_eta2, s->m_u, s->m_v, s->m_Du[1], s->m_Du[2], s->m_Dv[1], s->m_Dv[2], s->m_u2, s->m_v2, s->m_Du[0],
-               s->m_Dv[0], s->d_mt, s->d_cflp, n, k, s->Dt, s->sigma, s->c2, s->c4, s->SW->num_Gaussians > 0};
+               s->m_Dv[0], s->d_mt, s->d_cflp, n, k, s->Dt, s->sigma, s->c2, s->c4, s->SW->num_Gaussians > 0,
+               s->m_tr, s->m_Dtr[0], s->m_Dtr[1], s->m_Dtr[2], s->tracer_active};
     km_uv<SCH, HV><<<G, NTU, 0, s->st>>>(s->g, s->fl, ua);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 2], s->st);
     km_cfl_parts<<<1, 64, 0, s->st>>>(s->d_cflp, s->fl, slot ^ 1);
@@ -1304,16 +1315,19 @@ static int step_mixed(sw2_session *s) {
     frot(s->m_De);
     frot(s->m_Du);
     frot(s->m_Dv);
+    if (s->tracer_active) frot(s->m_Dtr);
     const double t = s->t + s->Dt;
     if (s->SW->num_Gaussians) {
-        // eta-only storms after u and v, then the height check (core.c:280-312)
+        // storms after u and v, then the tracer and height checks (core.c:280-312)
         if (s->perturbs_dirty) {
             k_zero_pert<<<(s->S + 255) / 256, 256, 0, s->st>>>(s->fl, n, s->S, s->pert);
             s->perturbs_dirty = 0;
         }
         int eta_changed = 0;
         if (host_perturbations(s, n, t, &eta_changed)) return -1;
-        k_check_mixed_eta<<<cells_grid(s->g), cells_block(), 0, s->st>>>(s->g, s->fl, s->m_eta, s->d_mt + MT_HBAR * s->g.R, n);
+        if (s->tracer_active)
+            k_check_mixed<<<cells_grid(s->g), cells_block(), 0, s->st>>>(s->g, s->fl, s->m_tr, nullptr, n, 2);
+        k_check_mixed<<<cells_grid(s->g), cells_block(), 0, s->st>>>(s->g, s->fl, s->m_eta, s->d_mt + MT_HBAR * s->g.R, n, 3);
     }
     s->n = n;
     s->t = t;
@@ -1499,10 +1513,10 @@ extern "C" int sw2_session_download(sw2_session *s, sw *SW) {
     HIPCHECK(hipStreamSynchronize(s->st));
     if (s->form == SW2_FORM_MIXED) {
         float *h = (float *)malloc(sizeof(float) * S), *hl = (float *)malloc(sizeof(float) * S);
-        MField src[3] = {s->m_eta, s->m_u, s->m_v};
-        double *dst[3] = {SW->eta, SW->u, SW->v};
+        MField src[4] = {s->m_eta, s->m_u, s->m_v, s->m_tr};
+        double *dst[4] = {SW->eta, SW->u, SW->v, SW->tracer};
         int rc = 0;
-        for (int f = 0; f < 3 && !rc; f++) {
+        for (int f = 0; f < (s->tracer_active ? 4 : 3) && !rc; f++) {
             if (hipMemcpy(h, src[f].hi, sizeof(float) * S, hipMemcpyDeviceToHost) != hipSuccess ||
                 hipMemcpy(hl, src[f].lo, sizeof(float) * S, hipMemcpyDeviceToHost) != hipSuccess) {
                 rc = -1;
@@ -1581,7 +1595,7 @@ extern "C" void sw2_session_destroy(sw2_session *s) {
         if (s->Dv[k]) (void)hipFree(s->Dv[k]);
         if (s->Dtr[k]) (void)hipFree(s->Dtr[k]);
     }
-    MField fall[] = {s->m_eta, s->m_u, s->m_v, s->m_eta2, s->m_u2, s->m_v2};
+    MField fall[] = {s->m_eta, s->m_u, s->m_v, s->m_eta2, s->m_u2, s->m_v2, s->m_tr};
     for (auto p : fall) {
         if (p.hi) (void)hipFree(p.hi);
         if (p.lo) (void)hipFree(p.lo);
@@ -1590,6 +1604,7 @@ extern "C" void sw2_session_destroy(sw2_session *s) {
         if (s->m_De[k]) (void)hipFree(s->m_De[k]);
         if (s->m_Du[k]) (void)hipFree(s->m_Du[k]);
         if (s->m_Dv[k]) (void)hipFree(s->m_Dv[k]);
+        if (s->m_Dtr[k]) (void)hipFree(s->m_Dtr[k]);
     }
     if (s->d_mt) (void)hipFree(s->d_mt);
     if (s->d_part) (void)hipFree(s->d_part);
@@ -1660,7 +1675,12 @@ static_assert(tile_64x16::NTE == NTE && tile_64x16::NTU == NTU && tile_64x8::NTE
 extern "C" size_t sw2_batch_footprint(const sw *SW, int form) {
     const size_t S = (size_t)(SW->nx + 4) * (SW->ny + 4), R = (size_t)SW->ny + 4;
     size_t f = (size_t)(SW2_NT + 2) * R * sizeof(double) + 3 * R * sizeof(double) + (3 * 1024 + 3) * sizeof(double) + sizeof(Flags);
-    if (form == SW2_FORM_MIXED) f += 12 * S * sizeof(float) + 9 * S * sizeof(float);
+    int tracer = 0;  /* the tracer can be nonzero: a storm injects it, or it is nonzero already */
+    for (int k = 0; k < SW->num_Gaussians; k++)
+        if (SW->Gaussians[k * N_COLS_GAUSS + 11] != 0) tracer = 1;
+    if (!tracer)
+        for (size_t q = 0; q < S && !tracer; q++) tracer = SW->tracer[q] != 0;
+    if (form == SW2_FORM_MIXED) f += (12 + 9 + (tracer ? 5 : 0)) * S * sizeof(float);
     else f += 24 * S * sizeof(double);
     if (SW->num_Gaussians) f += S * sizeof(double); /* the storms' marks */
     return f;
@@ -1792,6 +1812,7 @@ extern "C" int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id) {
         M.De[k] = s->m_De[k];
         M.Du[k] = s->m_Du[k];
         M.Dv[k] = s->m_Dv[k];
+        M.Dtr[k] = s->m_Dtr[k];
     }
     M.ntx = (SW->nx + BATCH_SHAPES[b->shape].tx - 1) / BATCH_SHAPES[b->shape].tx;
     M.nty = (SW->ny + BATCH_SHAPES[b->shape].ty - 1) / BATCH_SHAPES[b->shape].ty;
@@ -1806,6 +1827,9 @@ extern "C" int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id) {
     M.rest_x = s->rest_x;
     M.rest_y = s->rest_y;
     M.late = SW->num_Gaussians > 0;
+    M.tr = s->m_tr;
+    M.tr_on = s->tracer_active;
+    M.tsig2 = 2.0 * s->Dt * (s->tracer_dis);
     if (M.late) b->storms++;
     b->paused[slot] = 0;
     b->sess[slot] = s;
@@ -1992,7 +2016,9 @@ static int batch_storms(sw2_batch *b) {
         }
         int eta_changed = 0;
         if (host_perturbations_on(s, n, t, &eta_changed, b->st, eta_new)) return -1;
-        k_check_mixed_eta<<<cells_grid(s->g), cells_block(), 0, b->st>>>(s->g, s->fl, eta_new, s->d_mt + MT_HBAR * s->g.R, n);
+        if (s->tracer_active)
+            k_check_mixed<<<cells_grid(s->g), cells_block(), 0, b->st>>>(s->g, s->fl, s->m_tr, nullptr, n, 2);
+        k_check_mixed<<<cells_grid(s->g), cells_block(), 0, b->st>>>(s->g, s->fl, eta_new, s->d_mt + MT_HBAR * s->g.R, n, 3);
         s->n = n;
         s->t = t;
     }

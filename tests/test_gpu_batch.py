@@ -154,6 +154,39 @@ def test_batch_members_with_storms_equal_solo_runs():
     b.close()
 
 
+def test_batch_members_with_tracer_storms_equal_solo_runs():
+    """A member whose storms inject tracer (advected, dissipated, written by
+    the injections) beside a height-only storm member and a quiet one, on
+    the default and on 64 x 16 tiles: every member's eta, u, v, tracer and
+    storm marks equal its solo mixed run bit for bit."""
+    G1 = np.array([[0.0, 1e5, 200.0, 358.0, 20.0, 1, 0.4, 3e12, 1, 3.0, 8.0, 1.0],
+                   [50.0, 3000.0, 100.0, 150.0, -30.0, 0, -40.0, -1e12, 1, 2.5, 6.0, 1.0]])
+    G2 = np.array([[100.0, 3000.0, 100.0, 150.0, -30.0, 0, -40.0, -1e12, 1, 2.5, 6.0, 0.0]])
+    specs = [(256, 128, G1, 110), (512, 256, G2, 90), (384, 192, None, 100)]
+    for tiles in ((32, 16), (64, 16)):
+        b = Batch(capacity=4)
+        b.set_tiles(*tiles)
+        members = []
+        for nx, ny, G, n in specs:
+            kw = {} if G is None else {"Gaussians": G, "TracerDis": 1e-6}
+            p = cases.scaled_params(nx, ny, steps=n, lat=70.0, sch=0, **kw)  # one limiter per batch
+            bumps = cases.single_bump(nx + 1)
+            w, _ = cases.make_world(p, bumps)
+            members.append((b.add(w, n), p, bumps, n))
+        b.step(200)
+        b.sync()
+        for mid, p, bumps, n in members:
+            st = b.member(mid)
+            assert st["done"] and st["steps"] == n and st["failed_step"] == 0, st
+            w = b.download(mid)
+            ref = solo(p, bumps, n)
+            same(w, ref, f"member {mid} on {tiles} tiles")
+            for f in ("tracer", "perturbs"):
+                assert np.array_equal(w.field(f), ref.field(f)), (f, mid, tiles)
+        assert b.download(members[0][0]).interior("tracer").max() > 0.5
+        b.close()
+
+
 def test_all_members_held_then_released():
     """Every member held: the launches number no tiles, and the steps
     change nothing.  Released, one member mid-table held for a while (its

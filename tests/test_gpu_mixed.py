@@ -39,9 +39,9 @@ def run_pair(p, bumps, n):
     return w, o, s, fs, fo
 
 
-def report(label, w, ref):
+def report(label, w, ref, fields=("eta", "u", "v")):
     out = {}
-    for f in ("eta", "u", "v"):
+    for f in fields:
         l2, mx = errors(w.interior(f), ref(f))
         out[f] = (l2, mx)
         print(f"{label}: {f} relative L2 {l2:.3e} max {mx:.3e}", flush=True)
@@ -127,14 +127,48 @@ def test_mixed_refuses_cases_the_fused_step_does_not_take():
     w, _ = cases.make_world(p, cases.single_bump(1))
     with pytest.raises(SW2Error, match="mixed-precision form"):
         Session(w, FORM_MIXED)
-    # storms that inject tracer run on the fused FP64 step only (the mixed
-    # form carries no tracer)
-    G = np.array([[0.0, 3000.0, 100.0, 100.0, 10.0, 1, 1.0, 2e11, 1, 4.0, 10.0, 1.0]])
-    p = cases.scaled_params(128, 32, steps=5, lat=60.0, Gaussians=G)
+    # vortices run on the FP64 steps only
+    V = np.array([[100.0, 200.0, -20.0, 8.0, 5.0, -60.0, 2.0]])
+    p = cases.scaled_params(128, 32, steps=5, lat=60.0, vortices=V)
     w, _ = cases.make_world(p, cases.single_bump(1))
     assert Session(w).fused
     with pytest.raises(SW2Error, match="mixed-precision form"):
         Session(w, FORM_MIXED)
+
+
+def test_mixed_tracer_storms_against_fp64():
+    """Storms that inject tracer (perturbs.c:348-395) in the mixed form: the
+    tracer advected with the FP32 flux-limited scheme of the height plus its
+    non-conservative term and dissipation (calc.c:117-163), updated in FP64
+    as value + residual, and written by the injections.  One storm moves
+    across the periodic seam, one stops a third of the way in; tracer
+    dissipation on.
+    Against the FP64 GPU run (equal to the oracle): the same storm cells
+    marked, eta, u, v within the tolerance, and the tracer within it too.
+    (The second storm is a sink that also writes tracer.)"""
+    G = np.array([[0.0, 1e5, 200.0, 358.0, 20.0, 1, 0.4, 3e12, 1, 3.0, 8.0, 1.0],
+                  [300.0, 4000.0, 100.0, 150.0, -30.0, 0, -40.0, -1e12, 1, 2.5, 6.0, 1.0]])
+    # limiter sch 0: with SW2's sch 2 (MIN(r,1), unbounded below) the tracer's
+    # near-zero tails overflow the ratio and SW2 itself turns the tracer NaN
+    # at step 82 of this case (the oracle, equal to SW2, does too)
+    p = cases.scaled_params(512, 256, steps=300, lat=75.0, Gaussians=G, TracerDis=1e-6, sch=0)
+    bumps = cases.single_bump(33)
+    runs = {}
+    for form in (FORM_FP64, FORM_MIXED):
+        w, _ = cases.make_world(p, bumps)
+        s = Session(w, form)
+        assert s.form == form and s.fused
+        s.begin(0, p.t0)
+        s.step(300)
+        s.download()
+        runs[form] = (w, s.failed_step()[0])
+    (w64, f64), (w32, f32) = runs[FORM_FP64], runs[FORM_MIXED]
+    assert f64 == f32 == 0
+    assert np.array_equal(w32.field("perturbs"), w64.field("perturbs"))
+    assert w64.interior("tracer").max() > 0.5
+    e = report("mixed tracer storms 300 steps", w32, w64.interior, fields=("eta", "u", "v", "tracer"))
+    for f, (l2, mx) in e.items():
+        assert l2 <= TOL and mx <= TOL, (f, l2, mx)
 
 
 def test_mixed_eta_only_storms_against_fp64():

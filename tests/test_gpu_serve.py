@@ -85,10 +85,11 @@ def test_srpt_holds_long_runs_and_results_stay_solo():
 
 
 def test_served_requests_with_in_loop_storms_equal_solo_runs():
-    """Requests carrying in-loop storms (height only) beside requests
-    without: the storm members step without the graph and inject after
-    every step; every result, storm marks included, equals its solo run."""
-    storm = np.array([[0.0, 1e5, 150.0, 359.0, 15.0, 1, 0.5, 2e12, 1, 3.0, 8.0, 0.0]])
+    """Requests carrying in-loop storms (one injecting tracer, one height
+    only) beside requests without: the storm members step without the graph
+    and inject after every step; every result, tracer and storm marks
+    included, equals its solo run."""
+    storm = np.array([[0.0, 1e5, 150.0, 359.0, 15.0, 1, 0.5, 2e12, 1, 3.0, 8.0, 1.0]])
     sink = np.array([[50.0, 2500.0, 100.0, 120.0, -25.0, 0, -30.0, -8e11, 1, 2.5, 6.0, 0.0]])
     rng = np.random.default_rng(13)
     reqs = [(0.0, 256, 128, 80, cases.storm_bumps(rng), storm), (0.02, 512, 256, 60, cases.storm_bumps(rng)),
@@ -97,7 +98,7 @@ def test_served_requests_with_in_loop_storms_equal_solo_runs():
 
     def keep(k, batch, mid):
         w = batch.download(mid)
-        got[k] = {f: w.field(f).copy() for f in ("eta", "u", "v", "perturbs")}
+        got[k] = {f: w.field(f).copy() for f in ("eta", "u", "v", "tracer", "perturbs")}
 
     srv = Server(capacity=4, tick_steps=10, policy="srpt")
     st = srv.run(reqs, on_result=keep)
@@ -112,8 +113,9 @@ def test_served_requests_with_in_loop_storms_equal_solo_runs():
         s.step(steps)
         assert s.failed_step()[0] == 0
         s.download()
-        for f in ("eta", "u", "v", "perturbs"):
+        for f in ("eta", "u", "v", "tracer", "perturbs"):
             assert np.array_equal(got[k][f], w.field(f)), (k, f)
         s.close()
         w.close()
     assert got[0]["perturbs"].max() == 1.0
+    assert got[0]["tracer"].max() > 0.5
