@@ -312,9 +312,12 @@ void sw2_session_destroy(sw2_session *s);
  * launch per phase (by default one workgroup per tile; persistent
  * workgroups as a set launch shape); the steps are captured in a hipGraph
  * and replayed, captured again when the tile count changes (members joining,
- * leaving, held or released) while launching one workgroup per tile, and
- * launched one by one while a member has in-loop storms.  Members must be
- * cases the mixed form takes and share the limiter and hyperviscosity order.
+ * leaving, held or released) while launching one workgroup per tile.
+ * Members with in-loop storms step in the replayed graph too: before each
+ * replay the host evaluates their storms for the graph's steps and copies
+ * the cells to the device, where the graph's storm kernels inject them.
+ * Members must be cases the mixed form takes and share the limiter and
+ * hyperviscosity order.
  * ------------------------------------------------------------------------- */
 typedef struct sw2_batch sw2_batch;
 

@@ -23,6 +23,7 @@
 
 #include <utility>
 
+#include "sw2_batch.h"
 #include "sw2_internal.h"
 #include "sw2_xdiv.h"
 
@@ -36,80 +37,7 @@
     } while (0)
 
 namespace {
-
-constexpr int BX = 64;  // lanes along a row: one wavefront per row segment
-constexpr int BY = 4;   // rows per workgroup
-
-struct Grid {
-    int nx, ny, sx;     // interior size, row stride (nx+4)
-    int R;              // row-table length (ny+4)
-    const double *T;    // row tables, SW2_NT x R
-};
-
-__device__ __forceinline__ int IX(const Grid &g, int i, int j) { return (i + 1) + (j + 1) * g.sx; }
-__device__ __forceinline__ double TB(const Grid &g, int t, int j) { return g.T[t * g.R + (j + 1)]; }
-
-// fail word: step of the first failed guard (0 = none), kind, value
-struct Flags {
-    unsigned long long cfl[4];  // two slots of (max_x, max_y) bit patterns
-    int fail_step;
-    int fail_kind;              // 1 CFL, 2 tracer, 3 eta, 4 u, 5 v
-    // set once u or v holds a non-finite value: from then on an identically
-    // zero tracer no longer stays zero in SW2 (0*NaN), so its kernels run
-    int nonfinite;
-    // fields whose check failed at fail_step (bit = kind); SW2 checks tracer,
-    // eta, u, v in that order and reports the first (core.c:309-312)
-    int kind_mask;
-    double fail_val;
-    double last_dtcfl;
-    // fused path: nonfinite as it stood when this step began (kf_eta copies
-    // it), so that every block of kf_uv agrees on whether the tracer moves
-    int tracer_on;
-};
-
-// ------------------------------------------------------------ limiters
-// calc.c:54-63; MIN/MAX are the ternaries of sw.h:104-105
-#define DMIN(a, b) (((a) < (b)) ? (a) : (b))
-#define DMAX(a, b) (((a) > (b)) ? (a) : (b))
-template <int SCH>
-__device__ __forceinline__ double psi(double r) {
-    if constexpr (SCH == 0) return (DMAX(0, DMIN(DMIN((1 + r) / 2.0, 2.0 * r), 2.0)));
-    else if constexpr (SCH == 1) return (DMAX(DMAX(0, DMIN(2.0 * r, 1.0)), DMAX(0, DMIN(r, 2.0))));
-    else if constexpr (SCH == 2) return (DMIN(r, 1));
-    else if constexpr (SCH == 3) return ddiv((r + fabs(r)), (1.0 + r));
-    else if constexpr (SCH == 4) return (DMAX(0, DMIN(2.0 * r, DMIN(ddiv((1 + 2.0 * r), 3.0), 2.0))));
-    else if constexpr (SCH == 5) return ddiv((1.5 * ((r * r) + r)), ((r * r) + r + 1.0));
-    else if constexpr (SCH == 6) return ddiv(((r * r) + r), ((r * r) + 1.0));
-    else if constexpr (SCH == 7) return (DMAX(0, DMIN(DMIN(2.0, 2.0 * r), DMIN(0.25 + 0.75 * r, 0.75 + 0.25 * r))));
-    else if constexpr (SCH == 8) return (0);
-    else return (1.0);
-}
-
-// calc.c:43-44
-__device__ __forceinline__ double r2rP(double Bprev, double Bcurr, double Bpost) {
-    return (((Bpost - Bcurr) != 0) ? ddiv((Bcurr - Bprev), (Bpost - Bcurr)) : 0);
-}
-__device__ __forceinline__ double r2rM(double Bcurr, double Bpost, double Bnext) {
-    return (((Bpost - Bcurr) != 0) ? ddiv((Bnext - Bpost), (Bpost - Bcurr)) : 0);
-}
-// calc.c:48-49
-template <int SCH>
-__device__ __forceinline__ double b2bp(double B1, double B2, double rP, double CP) {
-    return (B1 + psi<SCH>(rP) * 0.5 * (1 - CP) * (B2 - B1));
-}
-template <int SCH>
-__device__ __forceinline__ double b2bm(double B1, double B2, double rM, double CM) {
-    return (B2 - psi<SCH>(rM) * 0.5 * (1 + CM) * (B2 - B1));
-}
-
-// Adams-Bashforth of do_AdamsBashforth (calc.c:680-697) via linop_scaf
-// (sppde_core.c:285-298): Euler, AB2, then AB3
-__device__ __forceinline__ double ab(int k, double d, double m1, double m2) {
-    if (k == 0) return d;
-    if (k == 1) return (3.0 / 2.0) * d + (-(1.0 / 2.0)) * m1;
-    double t = (-(4.0 / 3.0)) * m1 + (+(5.0 / 12.0)) * m2;
-    return (23.0 / 12.0) * d + 1 * t;
-}
+#include "sw2_base.inc"
 
 struct Ptrs {
     double *eta, *u, *v, *wx, *wy, *tr;
@@ -442,8 +370,6 @@ __global__ void k_lape_fix(Grid g, Flags *fl, double *eo) {
     eo[IX(g, i, 0)] = eo[IX(g, i, 1)];
 }
 
-// ------------------------------------------------ halo rules
-__device__ __forceinline__ int wrap(int a, int n) { return a < 1 ? a + n : (a > n ? a - n : a); }
 
 // single-map halo_update: every halo cell takes the value of its periodic
 // image in x and y (y first for interior columns, then x for all rows)
@@ -482,14 +408,6 @@ __global__ void k_channel(Grid g, Flags *fl, double *u, double *v) {
     u[IX(g, i, ny + 2)] = u[IX(g, i, ny)];
 }
 
-// ------------------------------------------------ check_stats_scaf guard (sppde_core.c:377-395)
-// SW2 crashes when min or max of the interior is outside [-1e10, 1e10] or not
-// finite.  Its min/max skip NaNs unless the first cell is NaN; hence: fail if
-// any non-NaN value is outside the range, or the cell (1,1) is NaN.
-__device__ __forceinline__ void flag_bad(Flags *fl, int n, int kind) {
-    int old = atomicCAS(&fl->fail_step, 0, n);
-    if (old == 0 || old == n) atomicOr(&fl->kind_mask, 1 << kind);
-}
 
 // the checks of one step all run, even after one of them failed: the step
 // completes, as SW2's does before check_stats_scaf aborts
@@ -549,7 +467,9 @@ __global__ void k_zero_pert(Flags *fl, int step, int S, double *f) {
 // the 64-member ensemble 2.45 -> 2.00 ms per step).  Each tile shape's
 // helpers and kernels live in a namespace of their own, so that
 // argument-dependent lookup (Grid and the argument records live in the
-// anonymous namespace) never sees two shapes' functions at once.
+// anonymous namespace) never sees two shapes' functions at once.  The batch
+// kernels, of every tile shape, are compiled in sw2_batch.hip, once per
+// shape, and launched through sw2_batch_phases_<shape>.
 #define SW2_TILE_X 32
 #define SW2_TILE_Y 16
 namespace tile_32x16 {
@@ -564,41 +484,25 @@ namespace tile_32x16 {
 #undef SW2_TILE_X
 #undef SW2_TILE_Y
 
-// The batch kernels again on further tile shapes, one namespace each; a
-// batch runs one shape at a time (sw2_batch_set_tiles; the autotuner picks
-// it per mix of member sizes).
-#define SW2_TILE_X 64
-#define SW2_TILE_Y 8
-namespace tile_64x8 {
-#include "sw2_tile.inc"
-#include "sw2_mixed.inc"
-}  // namespace tile_64x8
-#undef SW2_TILE_X
-#undef SW2_TILE_Y
-#define SW2_TILE_X 64
-#define SW2_TILE_Y 16
-namespace tile_64x16 {
-#include "sw2_tile.inc"
-#include "sw2_mixed.inc"
-}  // namespace tile_64x16
-#undef SW2_TILE_X
-#undef SW2_TILE_Y
-
 namespace {
 // storm injections on the mixed form (perturbs.c:348-395): the marks as
 // k_inject sets them, the eta increment on the height anomaly eta' (eta =
 // hbar(j) + eta') and, for storms that inject tracer, the tracer value
 // (tr_on), each written with its periodic images as the kernels write them
+__device__ __forceinline__ void inject_cell(const Grid &g, int q, double inc, double trv, int has_inc, MField eta, MField tr,
+                                            int tr_on, double *pert) {
+    pert[q] = 1.0;
+    const int c = q % g.sx - 1, j = q / g.sx - 1;
+    if (tr_on) mput_periodic<float, true>(g, tr.hi, c, j, mset_lo(tr, q, trv));
+    if (!has_inc) return;
+    mput_periodic<float, true>(g, eta.hi, c, j, mset_lo(eta, q, mget(eta, q) + inc));
+}
+
 __global__ void k_inject_mixed(Grid g, Flags *fl, int step, int n, const int *idx, const double *inc, const double *trv,
                                int has_inc, MField eta, MField tr, int tr_on, double *pert) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if ((fl->fail_step && (fl->fail_step != step || fl->fail_kind == 1)) || k >= n) return;
-    const int q = idx[k];
-    pert[q] = 1.0;
-    const int c = q % g.sx - 1, j = q / g.sx - 1;
-    if (tr_on) mput_periodic<float, true>(g, tr.hi, c, j, mset_lo(tr, q, trv[k]));
-    if (!has_inc) return;
-    mput_periodic<float, true>(g, eta.hi, c, j, mset_lo(eta, q, mget(eta, q) + inc[k]));
+    inject_cell(g, idx[k], inc[k], trv[k], has_inc, eta, tr, tr_on, pert);
 }
 
 // check_stats_scaf on a mixed form field (hbar(j) + the anomaly, for eta):
@@ -610,6 +514,71 @@ __global__ __launch_bounds__(BX *BY) void k_check_mixed(Grid g, Flags *fl, MFiel
     if (i > g.nx || j > g.ny) return;
     const double x = (hbar ? hbar[j + 1] : 0.0) + mget(f, IX(g, i, j));
     if ((x < -1e10) || (x > 1e10) || (i == 1 && j == 1 && x != x)) flag_bad(fl, n, kind);
+}
+
+// Storms of batch members inside a replayed graph.  The host evaluates a
+// window of steps ahead (the cells of every storm at each step, as a single
+// session does step by step) and writes the window table; the graph's three
+// kernels per step read their row of it.  They do what k_zero_pert,
+// k_inject_mixed and k_check_mixed do for a single mixed session, with the
+// same gates and the same operations per cell, in the same order, so that a
+// member's result is the one it has when run alone.
+constexpr int SMAXK = 8;  // storm lists per member and step (sw2_gaussians_sparse's maxk)
+struct SEntry {           // one storm member at one step of the window
+    Grid g;
+    Flags *fl;
+    int n;        // the step; 0: the member takes no step here
+    int zero, S;  // clear the marks of the previous injections (all S cells) first
+    double *pert;
+    MField eta, tr;  // the step's new height anomaly, and the tracer
+    int tr_on;
+    const double *hbar;
+    int nk;  // storm lists of this step
+    int cnt[SMAXK], has_inc[SMAXK];
+    const int *idx[SMAXK];
+    const double *inc[SMAXK], *trv[SMAXK];
+};
+
+// the marks (grid: blocks x entries)
+__global__ __launch_bounds__(256) void kbs_zero(const SEntry *__restrict__ w) {
+    const SEntry &E = w[blockIdx.y];
+    if (!E.n || !E.zero) return;
+    const Flags *fl = E.fl;
+    if (fl->fail_step && (fl->fail_step != E.n || fl->fail_kind == 1)) return;
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < E.S; q += gridDim.x * 256) E.pert[q] = 0;
+}
+
+// the injections: one workgroup per entry, its lists one after another as
+// the single session's launches run them (a cell in two storms takes the
+// second after the first)
+__global__ __launch_bounds__(1024) void kbs_inject(const SEntry *__restrict__ w) {
+    const SEntry &E = w[blockIdx.y];
+    if (!E.n) return;
+    const Flags *fl = E.fl;
+    if (fl->fail_step && (fl->fail_step != E.n || fl->fail_kind == 1)) return;
+    for (int k = 0; k < E.nk; k++) {
+        for (int c = threadIdx.x; c < E.cnt[k]; c += 1024)
+            inject_cell(E.g, E.idx[k][c], E.inc[k][c], E.trv[k][c], E.has_inc[k], E.eta, E.tr, E.tr_on, E.pert);
+        __syncthreads();
+    }
+}
+
+// the tracer and height checks after the injections
+__global__ __launch_bounds__(256) void kbs_check(const SEntry *__restrict__ w) {
+    const SEntry &E = w[blockIdx.y];
+    if (!E.n) return;
+    Flags *fl = E.fl;
+    if (fl->fail_step && fl->fail_step != E.n) return;
+    const Grid g = E.g;
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < g.nx * g.ny; q += gridDim.x * 256) {
+        const int i = 1 + q % g.nx, j = 1 + q / g.nx;
+        if (E.tr_on) {
+            const double x = 0.0 + mget(E.tr, IX(g, i, j));
+            if ((x < -1e10) || (x > 1e10) || (i == 1 && j == 1 && x != x)) flag_bad(fl, E.n, 2);
+        }
+        const double x = E.hbar[j + 1] + mget(E.eta, IX(g, i, j));
+        if ((x < -1e10) || (x > 1e10) || (i == 1 && j == 1 && x != x)) flag_bad(fl, E.n, 3);
+    }
 }
 
 // device reductions of SW2's energy diagnostics (energy.c:25-55) and of the
@@ -1277,27 +1246,33 @@ static int step_fused(sw2_session *s) {
 
 static int step_phased(sw2_session *s);
 
-template <int SCH, int HV>
+template <int SCH, int HV, bool TR>
 static void launch_mixed(sw2_session *s, int n, int k, int slot) {
     dim3 G((s->g.nx + FTX - 1) / FTX, (s->g.ny + FTY - 1) / FTY);
     MEtaArgs ea{s->m_eta, s->m_u, s->m_v, s->m_De[1], s->m_De[2], s->m_eta2, s->m_De[0], s->d_mt, n, k, slot,
                 s->Dt, 2.0 * s->Dt * (s->sigma), s->c2, s->c4, s->courant, s->rest_x, s->rest_y,
                 s->m_tr, s->m_Dtr[0], s->tracer_active, 2.0 * s->Dt * (s->tracer_dis)};
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i], s->st);
-    km_eta<SCH, HV><<<G, NTE, 0, s->st>>>(s->g, s->fl, ea);
+    km_eta<SCH, HV, TR><<<G, NTE, 0, s->st>>>(s->g, s->fl, ea);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 1], s->st);
     MUVArgs ua{s->m_eta2, s->m_u, s->m_v, s->m_Du[1], s->m_Du[2], s->m_Dv[1], s->m_Dv[2], s->m_u2, s->m_v2, s->m_Du[0],
                s->m_Dv[0], s->d_mt, s->d_cflp, n, k, s->Dt, s->sigma, s->c2, s->c4, s->SW->num_Gaussians > 0,
                s->m_tr, s->m_Dtr[0], s->m_Dtr[1], s->m_Dtr[2], s->tracer_active};
-    km_uv<SCH, HV><<<G, NTU, 0, s->st>>>(s->g, s->fl, ua);
+    km_uv<SCH, HV, TR><<<G, NTU, 0, s->st>>>(s->g, s->fl, ua);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 2], s->st);
     km_cfl_parts<<<1, 64, 0, s->st>>>(s->d_cflp, s->fl, slot ^ 1);
 }
 
+// the kernels with the tracer's code only for a world whose tracer runs
 template <int SCH>
 static void launch_mixed_hv(sw2_session *s, int n, int k, int slot) {
-    if (s->hvmode) launch_mixed<SCH, 2>(s, n, k, slot);
-    else launch_mixed<SCH, 0>(s, n, k, slot);
+    if (s->tracer_active) {
+        if (s->hvmode) launch_mixed<SCH, 2, true>(s, n, k, slot);
+        else launch_mixed<SCH, 0, true>(s, n, k, slot);
+    } else {
+        if (s->hvmode) launch_mixed<SCH, 2, false>(s, n, k, slot);
+        else launch_mixed<SCH, 0, false>(s, n, k, slot);
+    }
 }
 
 static int step_mixed(sw2_session *s) {
@@ -1655,7 +1630,22 @@ struct sw2_batch {
     hipGraph_t graph;
     hipGraphExec_t gexec;
     int gsteps, blocks_eta, blocks_uv, table_dirty;
-    int storms;   // members with in-loop storms: steps launched one by one, storms after each
+    int storms;   // members with in-loop storms (their storms run after each step)
+    int tracers;  // members whose tracer runs: the kernels with the tracer's code step the batch
+    int graph_tr; // whether the captured graph holds those kernels
+    // storms in a replayed graph: the graph's storm kernels cover graph_nl
+    // members (the storm members when it was captured) and read the window
+    // table d_wt (gsteps x graph_nl entries) and the cell lists in d_cells,
+    // which the host fills ahead of each replay through pinned staging
+    // (two sets, used alternately, each reused once its copies have run)
+    int graph_nl, wt_cap;
+    SEntry *d_wt, *h_wt;
+    char *d_cells, *h_build;
+    size_t cells_cap, build_cap;
+    char *h_stage[2];
+    size_t stage_cap[2];
+    hipEvent_t stage_ev[2];
+    int stage_i;
     int *paused;  // host copy of the members' paused flags
     int *h_tmap, *d_tmap;  // member slot of every flat tile, host and device
     int tmap_cap;          // entries of both
@@ -1667,10 +1657,9 @@ struct sw2_batch {
 struct BatchShape {
     int tx, ty;
 };
-static const BatchShape BATCH_SHAPES[] = {{FTX, FTY}, {tile_64x8::FTX, tile_64x8::FTY}, {tile_64x16::FTX, tile_64x16::FTY}};
+static const BatchShape BATCH_SHAPES[] = {{32, 16}, {64, 8}, {64, 16}};
 constexpr int NBATCH_SHAPES = 3;
-static_assert(tile_64x16::NTE == NTE && tile_64x16::NTU == NTU && tile_64x8::NTE == NTE && tile_64x8::NTU == NTU,
-              "every tile shape launches the same workgroup sizes");
+static_assert(FTX == 32 && FTY == 16, "the single-run kernels use the first batch shape");
 
 extern "C" size_t sw2_batch_footprint(const sw *SW, int form) {
     const size_t S = (size_t)(SW->nx + 4) * (SW->ny + 4), R = (size_t)SW->ny + 4;
@@ -1831,6 +1820,7 @@ extern "C" int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id) {
     M.tr_on = s->tracer_active;
     M.tsig2 = 2.0 * s->Dt * (s->tracer_dis);
     if (M.late) b->storms++;
+    if (M.tr_on) b->tracers++;
     b->paused[slot] = 0;
     b->sess[slot] = s;
     b->foot[slot] = foot;
@@ -1850,42 +1840,16 @@ extern "C" int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id) {
 static void batch_launch_step(sw2_batch *b) {
     const int tiles = b->h_hdr.total_tiles > 0 ? b->h_hdr.total_tiles : 1;
     const int ge = b->blocks_eta ? b->blocks_eta : tiles, gu = b->blocks_uv ? b->blocks_uv : tiles;
-// persistent workgroups (blocks_eta / blocks_uv) exist for the default tiles only
-#define BATCH_PHASE(S, HV)                                                                                \
-    do {                                                                                                  \
-        if (b->shape == 1) {                                                                              \
-            tile_64x8::kb_eta<S, HV><<<ge, NTE, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);                 \
-            tile_64x8::kb_uv<S, HV><<<gu, NTU, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);                  \
-        } else if (b->shape == 2) {                                                                       \
-            tile_64x16::kb_eta<S, HV><<<ge, NTE, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);                \
-            tile_64x16::kb_uv<S, HV><<<gu, NTU, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);                 \
-        } else {                                                                                          \
-            if (b->blocks_eta) kbp_eta<S, HV><<<ge, NTE, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);        \
-            else kb_eta<S, HV><<<ge, NTE, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);                       \
-            if (b->blocks_uv) kbp_uv<S, HV><<<gu, NTU, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);          \
-            else kb_uv<S, HV><<<gu, NTU, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);                        \
-        }                                                                                                 \
-    } while (0)
-#define BATCH_LAUNCH(S)                                                                                   \
-    case S:                                                                                               \
-        if (b->hvmode) BATCH_PHASE(S, 2);                                                                 \
-        else BATCH_PHASE(S, 0);                                                                           \
-        break;
-    switch (b->sch) {
-        BATCH_LAUNCH(0)
-        BATCH_LAUNCH(1)
-        BATCH_LAUNCH(2)
-        BATCH_LAUNCH(3)
-        BATCH_LAUNCH(4)
-        BATCH_LAUNCH(5)
-        BATCH_LAUNCH(6)
-        BATCH_LAUNCH(7)
-        BATCH_LAUNCH(8)
-        default:
-            BATCH_LAUNCH(9)
-    }
-#undef BATCH_LAUNCH
-#undef BATCH_PHASE
+    // the kernels with the tracer's code only while a member's tracer runs;
+    // persistent workgroups exist for the 32 x 16 tiles only
+    const int hv = b->hvmode ? 2 : 0, tr = b->tracers > 0;
+    if (b->shape == 1)
+        sw2_batch_phases_64x8(b->d_mem, b->d_st, b->d_hdr, b->sch, hv, tr, ge, gu, 0, 0, b->st);
+    else if (b->shape == 2)
+        sw2_batch_phases_64x16(b->d_mem, b->d_st, b->d_hdr, b->sch, hv, tr, ge, gu, 0, 0, b->st);
+    else
+        sw2_batch_phases_32x16(b->d_mem, b->d_st, b->d_hdr, b->sch, hv, tr, ge, gu, b->blocks_eta != 0,
+                               b->blocks_uv != 0, b->st);
     kb_adv<<<(b->cap + 3) / 4, 256, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);
 }
 
@@ -1921,10 +1885,52 @@ static int batch_upload(sw2_batch *b) {
     return 0;
 }
 
+static void batch_drop_graph(sw2_batch *b) {
+    if (!b->gexec) return;
+    (void)hipStreamSynchronize(b->st);
+    (void)hipGraphExecDestroy(b->gexec);
+    (void)hipGraphDestroy(b->graph);
+    b->gexec = NULL;
+    b->graph = NULL;
+}
+
+// the storm kernels of window step q, for the graph_nl storm members
+static void batch_launch_storms(sw2_batch *b, int q) {
+    const int nl = b->graph_nl;
+    const SEntry *w = b->d_wt + (size_t)q * nl;
+    kbs_zero<<<dim3(256, nl), 256, 0, b->st>>>(w);
+    kbs_inject<<<dim3(1, nl), 1024, 0, b->st>>>(w);
+    kbs_check<<<dim3(256, nl), 256, 0, b->st>>>(w);
+}
+
 static int batch_capture(sw2_batch *b) {
-    if (b->gexec) return 0;
+    const int nl = b->storms;
+    if (b->gexec && b->graph_tr == (b->tracers > 0) && b->graph_nl == nl) return 0;
+    // captured with the other kernel variant (tracer code in or out) or for
+    // another number of storm members
+    batch_drop_graph(b);
+    if (nl && b->gsteps * nl > b->wt_cap) {
+        const int cap = b->gsteps * nl;
+        SEntry *h = (SEntry *)realloc(b->h_wt, sizeof(SEntry) * (size_t)cap);
+        if (!h) {
+            sw2_set_error("out of host memory for the storm window table");
+            return -1;
+        }
+        b->h_wt = h;
+        HIPCHECK(hipStreamSynchronize(b->st));
+        if (b->d_wt) HIPCHECK(hipFree(b->d_wt));
+        b->d_wt = NULL;
+        b->wt_cap = 0;
+        HIPCHECK(hipMalloc((void **)&b->d_wt, sizeof(SEntry) * (size_t)cap));
+        b->wt_cap = cap;
+    }
+    b->graph_tr = b->tracers > 0;
+    b->graph_nl = nl;
     HIPCHECK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-    for (int q = 0; q < b->gsteps; q++) batch_launch_step(b);
+    for (int q = 0; q < b->gsteps; q++) {
+        batch_launch_step(b);
+        if (nl) batch_launch_storms(b, q);
+    }
     HIPCHECK(hipStreamEndCapture(b->st, &b->graph));
     HIPCHECK(hipGraphInstantiate(&b->gexec, b->graph, NULL, NULL, 0));
     return 0;
@@ -2025,22 +2031,134 @@ static int batch_storms(sw2_batch *b) {
     return 0;
 }
 
+// the host half of a replayed window of G steps with storm members: each
+// storm member's storms at each step it will take in the window, evaluated
+// on the host as a single mixed session evaluates them step by step (SW2's
+// expressions and libm), gathered into the window table and one block of
+// cell lists, and copied to the device ahead of the graph that reads them.
+// The members' step counts, times and mark states advance here, as
+// batch_storms advances them step by step; a member that fails a guard in
+// the window is held back on the device by its guard record (the kernels'
+// gates), as it is in the direct launches.
+static int batch_storm_window(sw2_batch *b, int G) {
+    const int nl = b->graph_nl;
+    size_t used = 0;
+    auto take = [&](size_t bytes) -> long {  // 8-aligned room in the host assembly, as an offset
+        const size_t off = (used + 7) & ~(size_t)7;
+        if (off + bytes > b->build_cap) {
+            const size_t cap = (off + bytes) * 2 + (1 << 16);
+            char *nb = (char *)realloc(b->h_build, cap);
+            if (!nb) return -1;
+            b->h_build = nb;
+            b->build_cap = cap;
+        }
+        used = off + bytes;
+        return (long)off;
+    };
+    for (int q = 0; q < G; q++) {
+        int e = 0;
+        for (int m = 0; m < b->cap; m++) {
+            sw2_session *s = b->sess[m];
+            const BMember &M = b->h_mem[m];
+            if (!s || !M.late) continue;
+            SEntry &E = b->h_wt[(size_t)q * nl + e++];
+            memset(&E, 0, sizeof E);
+            if (b->paused[m] || s->n >= M.nlast) continue;
+            const int n = s->n + 1;
+            const double t = s->t + s->Dt;
+            E.g = s->g;
+            E.fl = s->fl;
+            E.n = n;
+            E.S = s->S;
+            E.pert = s->pert;
+            E.eta = M.eta[((n - M.nstart - 1) & 1) ^ 1];
+            E.tr = s->m_tr;
+            E.tr_on = s->tracer_active;
+            E.hbar = s->d_mt + MT_HBAR * s->g.R;
+            E.zero = s->perturbs_dirty;
+            s->perturbs_dirty = 0;
+            const int K = sw2_gaussians_sparse(s->SW, s->Dt, t, SMAXK, s->inj);
+            if (K < 0) return -1;
+            for (int k = 0; k < K; k++) {
+                const sw2_inj &x = s->inj[k];
+                if (x.n == 0) continue;
+                const long oi = take(sizeof(int) * (size_t)x.n), oc = oi < 0 ? -1 : take(sizeof(double) * (size_t)x.n),
+                           ot = oc < 0 ? -1 : take(sizeof(double) * (size_t)x.n);
+                if (ot < 0) {
+                    sw2_set_error("out of host memory for the storm cells of a window");
+                    return -1;
+                }
+                memcpy(b->h_build + oi, x.idx, sizeof(int) * (size_t)x.n);
+                memcpy(b->h_build + oc, x.inc, sizeof(double) * (size_t)x.n);
+                memcpy(b->h_build + ot, x.trv, sizeof(double) * (size_t)x.n);
+                const int c = E.nk++;
+                E.cnt[c] = x.n;
+                E.has_inc[c] = x.has_inc;
+                // offsets into the cell block until it has its device address
+                E.idx[c] = (const int *)(uintptr_t)oi;
+                E.inc[c] = (const double *)(uintptr_t)oc;
+                E.trv[c] = (const double *)(uintptr_t)ot;
+                s->perturbs_dirty = 1;
+            }
+            s->n = n;
+            s->t = t;
+        }
+    }
+    if (used > b->cells_cap) {
+        HIPCHECK(hipStreamSynchronize(b->st));  // the last replay may still read the old block
+        if (b->d_cells) HIPCHECK(hipFree(b->d_cells));
+        b->d_cells = NULL;
+        b->cells_cap = 0;
+        const size_t cap = used + used / 2 + (1 << 16);
+        HIPCHECK(hipMalloc((void **)&b->d_cells, cap));
+        b->cells_cap = cap;
+    }
+    const size_t nent = (size_t)G * nl;
+    for (size_t k = 0; k < nent; k++) {
+        SEntry &E = b->h_wt[k];
+        for (int c = 0; c < E.nk; c++) {
+            E.idx[c] = (const int *)(b->d_cells + (uintptr_t)E.idx[c]);
+            E.inc[c] = (const double *)(b->d_cells + (uintptr_t)E.inc[c]);
+            E.trv[c] = (const double *)(b->d_cells + (uintptr_t)E.trv[c]);
+        }
+    }
+    const int si = b->stage_i;
+    b->stage_i ^= 1;
+    const size_t tab = sizeof(SEntry) * nent, need = tab + used;
+    if (b->stage_ev[si]) HIPCHECK(hipEventSynchronize(b->stage_ev[si]));
+    else HIPCHECK(hipEventCreateWithFlags(&b->stage_ev[si], hipEventDisableTiming));
+    if (need > b->stage_cap[si]) {
+        if (b->h_stage[si]) HIPCHECK(hipHostFree(b->h_stage[si]));
+        b->h_stage[si] = NULL;
+        b->stage_cap[si] = 0;
+        const size_t cap = need + need / 2 + (1 << 16);
+        HIPCHECK(hipHostMalloc((void **)&b->h_stage[si], cap, hipHostMallocDefault));
+        b->stage_cap[si] = cap;
+    }
+    memcpy(b->h_stage[si], b->h_wt, tab);
+    if (used) memcpy(b->h_stage[si] + tab, b->h_build, used);
+    HIPCHECK(hipMemcpyAsync(b->d_wt, b->h_stage[si], tab, hipMemcpyHostToDevice, b->st));
+    if (used) HIPCHECK(hipMemcpyAsync(b->d_cells, b->h_stage[si] + tab, used, hipMemcpyHostToDevice, b->st));
+    HIPCHECK(hipEventRecord(b->stage_ev[si], b->st));
+    return 0;
+}
+
 extern "C" int sw2_batch_step(sw2_batch *b, int nsteps) {
     if (b->nmem == 0 || nsteps <= 0) return 0;
     if (batch_upload(b)) return -1;
-    if (b->storms) {
-        for (int q = 0; q < nsteps; q++) {
-            batch_launch_step(b);
-            if (batch_storms(b)) return -1;
-        }
-        HIPCHECK(hipGetLastError());
-        return 0;
-    }
-    // the graph is captured only when this call replays it
+    // whole graphs of gsteps steps (members with storms included: the host
+    // fills each replay's storm window ahead of it), the remainder launched
+    // directly, storms after each step
     if (nsteps >= b->gsteps && batch_capture(b)) return -1;
     int q = 0;
-    for (; q + b->gsteps <= nsteps; q += b->gsteps) HIPCHECK(hipGraphLaunch(b->gexec, b->st));
-    for (; q < nsteps; q++) batch_launch_step(b);
+    for (; q + b->gsteps <= nsteps; q += b->gsteps) {
+        if (b->graph_nl && batch_storm_window(b, b->gsteps)) return -1;
+        HIPCHECK(hipGraphLaunch(b->gexec, b->st));
+    }
+    for (; q < nsteps; q++) {
+        batch_launch_step(b);
+        if (b->storms && batch_storms(b)) return -1;
+    }
     HIPCHECK(hipGetLastError());
     return 0;
 }
@@ -2138,6 +2256,7 @@ extern "C" int sw2_batch_remove(sw2_batch *b, int id) {
     BState z{0, 1, 0};
     HIPCHECK(hipMemcpy(b->d_st + id, &z, sizeof z, hipMemcpyHostToDevice));
     if (b->h_mem[id].late) b->storms--;
+    if (b->h_mem[id].tr_on) b->tracers--;
     memset(&b->h_mem[id], 0, sizeof(BMember));
     batch_retile(b);
     // the member's device arrays are released only after the table that
@@ -2168,6 +2287,14 @@ extern "C" void sw2_batch_destroy(sw2_batch *b) {
     if (b->d_hdr) (void)hipFree(b->d_hdr);
     if (b->d_tmap) (void)hipFree(b->d_tmap);
     free(b->h_tmap);
+    if (b->d_wt) (void)hipFree(b->d_wt);
+    if (b->d_cells) (void)hipFree(b->d_cells);
+    free(b->h_wt);
+    free(b->h_build);
+    for (int k = 0; k < 2; k++) {
+        if (b->h_stage[k]) (void)hipHostFree(b->h_stage[k]);
+        if (b->stage_ev[k]) (void)hipEventDestroy(b->stage_ev[k]);
+    }
     if (b->st) (void)hipStreamDestroy(b->st);
     free(b->sess);
     free(b->foot);

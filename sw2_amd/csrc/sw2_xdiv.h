@@ -85,7 +85,7 @@ SW2_HD double sw2_nearest(double a, double b, double q) {
 
 /* every case ddiv's fast path does not take */
 #if defined(__HIPCC__)
-__host__ __device__ __noinline__
+static __host__ __device__ __noinline__
 #else
 static
 #endif
