@@ -280,17 +280,18 @@ def fp64_step_ms(nx, ny, steps=40):
 
 def fp64_run(p, bumps, steps):
     """One run in the FP64 form, which equals the oracle bit for bit: its
-    wall time (world upload to the last step's completion, the world built
-    beforehand), failed step and final eta, u, v interiors."""
+    wall time (world upload to the result's download into the world's host
+    arrays, the world built beforehand, as a served or batched result is
+    timed), failed step and final eta, u, v interiors."""
     from sw2_amd import FORM_FP64, Session, cases
     w, _ = cases.make_world(p, bumps)
     t0 = time.perf_counter()
     s = Session(w, FORM_FP64)
     s.begin(0, p.t0)
     s.timed_steps(steps)  # returns once the last step has completed
-    wall = time.perf_counter() - t0
     failed = s.failed_step()[0]
     s.download()
+    wall = time.perf_counter() - t0
     s.close()
     out = {f: w.interior(f).copy() for f in FIELDS}
     w.close()
@@ -326,32 +327,26 @@ def bench_ensemble(n, steps, tune_launch=True):
     if shape:
         tune.apply(b, shape)
     shape = tuple(b.launch()) + tuple(b.tiles())
-    worlds, ids = [], []
+    progress(f"ensemble: building {n} worlds")
+    worlds = [cases.make_world(p, bumps)[0] for p, bumps in specs]
+    # timed as the FP64 runs are: from the members' uploads to their results
+    # downloaded into the worlds' host arrays (the worlds built beforehand)
+    progress(f"ensemble: admitting {n} members, {steps} steps, results downloaded")
     t0 = time.perf_counter()
-    for p, bumps in specs:
-        w, _ = cases.make_world(p, bumps)
-        worlds.append(w)
-        ids.append(b.add(w, steps))
+    ids = [b.add(w, steps) for w in worlds]
     t_admit = time.perf_counter() - t0
-    progress(f"ensemble: {steps} steps")
-    warm = min(shape[2], steps // 2)
-    b.step(warm)  # capture the graph (counted steps)
+    b.step(steps)  # the graph's capture included
     b.sync()
-    t0 = time.perf_counter()
-    b.step(steps - warm)
-    b.sync()
-    wall = time.perf_counter() - t0
+    t_steps = time.perf_counter() - t0 - t_admit
     fin = b.finished()
     failed = sum(1 for m in fin if b.member(m)["failed_step"])
-    got = []
-    for mid in ids:
-        w = b.download(mid)
-        got.append({f: w.interior(f).copy() for f in FIELDS})
+    got = [b.download(mid) for mid in ids]
+    wall = time.perf_counter() - t0
+    got = [{f: w.interior(f).copy() for f in FIELDS} for w in got]
     b.close()
     for w in worlds:
         w.close()
-    cells_steps = sum(float(nx) * ny for nx, ny in sizes) * (steps - warm)
-    wall_all = wall * steps / (steps - warm)
+    cells_steps = sum(float(nx) * ny for nx, ny in sizes) * steps
     progress(f"ensemble: every member alone in the FP64 form, {steps} steps")
     t64, errs, f64_failed = 0.0, [], 0
     for (p, bumps), res in zip(specs, got):
@@ -360,13 +355,15 @@ def bench_ensemble(n, steps, tune_launch=True):
         f64_failed += fl != 0
         errs.append((f"{p.nx}x{p.ny}", {f: rel_errors(res[f], ref[f]) for f in FIELDS}))
     return {"members": n, "steps": steps, "sizes": {f"{a}x{b_}": sizes.count((a, b_)) for a, b_ in sorted(set(sizes))},
-            "runs_per_hour": n / wall_all * 3600.0, "cell_updates_per_s": cells_steps / wall, "wall_s": wall_all,
+            "runs_per_hour": n / wall * 3600.0, "cell_updates_per_s": cells_steps / t_steps, "wall_s": wall,
+            "steps_s": t_steps, "download_s": wall - t_admit - t_steps,
             "completed": len(fin), "failed": failed,
             "launch": {"blocks_eta": shape[0], "blocks_uv": shape[1], "steps_per_graph": shape[2],
                        "tiles": [shape[3], shape[4]], "tuning": rec},
             "admit_s": t_admit, "fp64_wall_s": t64, "fp64_failed": f64_failed,
-            "fp64_runs_per_hour": n / t64 * 3600.0, "vs_fp64_runs_per_hour": (n / wall_all) / (n / t64),
-            "fp64_note": "every member run alone in the FP64 form, one after another (measured wall time)",
+            "fp64_runs_per_hour": n / t64 * 3600.0, "vs_fp64_runs_per_hour": (n / wall) / (n / t64),
+            "fp64_note": "every member run alone in the FP64 form, one after another (measured wall time, upload to "
+                         "download, as the batch's wall_s)",
             "errors_vs_fp64": error_summary(errs)}
 
 
@@ -381,8 +378,7 @@ def bench_served(n):
     got = {}
 
     def keep(k, batch, mid):
-        w = batch.download(mid)
-        got[k] = {f: w.interior(f).copy() for f in FIELDS}
+        got[k] = batch.download(mid)  # the result in its world's host arrays (kept; read after the run)
 
     # the launch and tile shape tuned on a batch of the stream's size mix
     from sw2_amd import tune
@@ -403,7 +399,7 @@ def bench_served(n):
         w64, fl, ref = fp64_run(cases.member_params(nx, ny, steps), bumps, steps)
         durations[k] = w64
         if k in got:
-            errs.append((f"{k}:{nx}x{ny}:{steps}", {f: rel_errors(got[k][f], ref[f]) for f in FIELDS}))
+            errs.append((f"{k}:{nx}x{ny}:{steps}", {f: rel_errors(got[k].interior(f), ref[f]) for f in FIELDS}))
     for k, r in sorted(enumerate(reqs), key=lambda kr: kr[1][0]):
         t = max(t, r[0]) + durations[k]
     st.update({"fp64_fcfs_wall_s": t, "fp64_runs_per_hour": len(reqs) / t * 3600.0,

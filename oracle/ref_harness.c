@@ -18,12 +18,15 @@
  *     read_vortices / read_Gaussians / read_wind, the halo updates) is SW2's.
  *   - SW2 reads memory it never wrote: windy (init.c:204,234) and the psi /
  *     xi halos of laplacian_eta (laplacian.c:117-127).  In SW2's own runs of
- *     grids above glibc's mmap threshold those reads see fresh zero pages;
- *     the harness makes that hold for every size by having malloc map every
- *     block of 4 KiB or more (mallopt), so repeated runs in one process read
- *     zeros as the first run does.
+ *     grids above glibc's mmap threshold those reads see fresh zero pages.
+ *     The harness makes that hold for every size and every state of the
+ *     process's heap: the library defines malloc as calloc (below), and
+ *     -Bsymbolic binds SW2's calls to that definition.  (Mapping every block
+ *     of 4 KiB or more with mallopt was not enough: a request that a freed
+ *     chunk of the heap can serve is served from it, with whatever an
+ *     earlier user left there, and in a test process that held SW2's reads
+ *     to the history of the process.)
  */
-#include <malloc.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -35,12 +38,15 @@
 
 static int g_started = 0;
 
+/* SW2's allocations, zero-filled (see above).  Only calls made from inside
+ * this library bind here (-Bsymbolic; the library is not loaded into the
+ * global scope); the C library's own calls keep its malloc. */
+void *malloc(size_t n) { return calloc(1, n); }
+
 /* MPI (a singleton unless started by mpiexec) and SW2's output channel */
 int ref_start(const char *outdir) {
     int on = 0;
     if (g_started) return 0;
-    mallopt(M_MMAP_THRESHOLD, 4096);
-    mallopt(M_MMAP_MAX, 1 << 30);
     MPI_Initialized(&on);
     if (!on) checkr(MPI_Init(NULL, NULL), "init");
     /* mode 2: rank 0 prints to stdout, the others print nothing */

@@ -1633,6 +1633,7 @@ struct sw2_batch {
     int storms;   // members with in-loop storms (their storms run after each step)
     int tracers;  // members whose tracer runs: the kernels with the tracer's code step the batch
     int graph_tr; // whether the captured graph holds those kernels
+    int graph_stale;  // the tile count changed since the capture (one workgroup per tile)
     // storms in a replayed graph: the graph's storm kernels cover graph_nl
     // members (the storm members when it was captured) and read the window
     // table d_wt (gsteps x graph_nl entries) and the cell lists in d_cells,
@@ -1742,14 +1743,10 @@ static void batch_retile(sw2_batch *b) {
         b->h_mem[m].tile0 = t;
         if (!b->paused[m]) t += b->h_mem[m].ntx * b->h_mem[m].nty;
     }
-    // a graph launching one workgroup per tile holds the old tile count
-    if (t != b->h_hdr.total_tiles && (b->blocks_eta == 0 || b->blocks_uv == 0) && b->gexec) {
-        (void)hipStreamSynchronize(b->st);
-        (void)hipGraphExecDestroy(b->gexec);
-        (void)hipGraphDestroy(b->graph);
-        b->gexec = NULL;
-        b->graph = NULL;
-    }
+    // a graph launching one workgroup per tile holds the old tile count: it
+    // is captured again at the next step (a replay may still be running, so
+    // it is not destroyed here)
+    if (t != b->h_hdr.total_tiles && (b->blocks_eta == 0 || b->blocks_uv == 0) && b->gexec) b->graph_stale = 1;
     b->h_hdr.total_tiles = t;
     b->table_dirty = 1;
 }
@@ -1827,12 +1824,14 @@ extern "C" int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id) {
     b->used += foot;
     b->nmem++;
     batch_retile(b);
-    // a fresh guard record and the CFL maxima of the member's first step
-    BState z{0, 0, 0};
+    // a fresh guard record, the CFL maxima of the member's first step and
+    // its step state, queued behind whatever the batch stream still runs:
+    // admission does not wait for the device (the session's uploads above
+    // have completed), and the member's first step is the batch's next
     HIPCHECK(hipMemsetAsync(s->fl, 0, sizeof(Flags), b->st));
     km_cfl<<<256, 256, 0, b->st>>>(s->g, s->fl, s->m_u, s->m_v, 1);
-    HIPCHECK(hipMemcpyAsync(b->d_st + slot, &z, sizeof z, hipMemcpyHostToDevice, b->st));
-    HIPCHECK(hipStreamSynchronize(b->st));
+    kb_join<<<1, 64, 0, b->st>>>(b->d_st, slot);
+    HIPCHECK(hipGetLastError());
     *id = slot;
     return 0;
 }
@@ -1886,6 +1885,7 @@ static int batch_upload(sw2_batch *b) {
 }
 
 static void batch_drop_graph(sw2_batch *b) {
+    b->graph_stale = 0;
     if (!b->gexec) return;
     (void)hipStreamSynchronize(b->st);
     (void)hipGraphExecDestroy(b->gexec);
@@ -1905,9 +1905,9 @@ static void batch_launch_storms(sw2_batch *b, int q) {
 
 static int batch_capture(sw2_batch *b) {
     const int nl = b->storms;
-    if (b->gexec && b->graph_tr == (b->tracers > 0) && b->graph_nl == nl) return 0;
-    // captured with the other kernel variant (tracer code in or out) or for
-    // another number of storm members
+    if (b->gexec && !b->graph_stale && b->graph_tr == (b->tracers > 0) && b->graph_nl == nl) return 0;
+    // captured for another tile count, with the other kernel variant (tracer
+    // code in or out) or for another number of storm members
     batch_drop_graph(b);
     if (nl && b->gsteps * nl > b->wt_cap) {
         const int cap = b->gsteps * nl;

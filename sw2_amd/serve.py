@@ -74,6 +74,30 @@ class Server:
             if wq[2] is None:
                 wq[2] = self.pool.submit(_build, wq[1])
 
+    def _admit(self, waiting, live, left, rejected, t0):
+        """Admit waiting requests in arrival order.  With runs on the device
+        only worlds already built are taken; with none the head is waited
+        for.  A request that does not fit even an empty batch is rejected."""
+        while waiting:
+            self._prefetch(waiting)
+            k, req, fut = waiting[0]
+            ta, nx, ny, steps = req[:4]
+            if live and not fut.done():
+                break
+            w = fut.result()
+            mid = self.batch.add(w, steps)
+            if mid is None and not live:
+                # no room even in an empty batch: it can never run here
+                waiting.popleft()
+                w.close()
+                rejected.append(k)
+                continue
+            if mid is None:
+                break
+            waiting.popleft()
+            live[mid] = (k, ta, time.perf_counter() - t0)
+            left[mid] = (steps, nx * ny, ((nx + 63) // 64 + 1) * ((ny + 7) // 8))
+
     def run(self, requests, on_result=None, max_wall=None, log=None):
         """Serve `requests` = [(t_arrival_s, nx, ny, steps, bumps[, storms]), ...] in
         real time; returns the statistics (and calls on_result(k, batch,
@@ -99,35 +123,23 @@ class Server:
                 waiting.append([reqs[nxt][0], reqs[nxt][1], None])
                 nxt += 1
             self._prefetch(waiting)
-            # admit in arrival order; with runs on the device only worlds
-            # already built are taken, with none the head is waited for
-            while waiting:
-                self._prefetch(waiting)
-                k, req, fut = waiting[0]
-                ta, nx, ny, steps = req[:4]
-                if live and not fut.done():
-                    break
-                w = fut.result()
-                mid = self.batch.add(w, steps)
-                if mid is None and not live:
-                    # no room even in an empty batch: it can never run here
-                    waiting.popleft()
-                    w.close()
-                    rejected.append(k)
-                    continue
-                if mid is None:
-                    break
-                waiting.popleft()
-                live[mid] = (k, ta, time.perf_counter() - t0)
-                left[mid] = (steps, nx * ny, ((nx + 63) // 64 + 1) * ((ny + 7) // 8))
-            self._prefetch(waiting)
             if not live:
-                if nxt < len(reqs):
-                    time.sleep(max(0.0, min(0.05, reqs[nxt][1][0] - (time.perf_counter() - t0))))
-                continue
+                # nothing on the device: admit in arrival order, waiting for
+                # the head's world to be built
+                self._admit(waiting, live, left, rejected, t0)
+                self._prefetch(waiting)
+                if not live:
+                    if nxt < len(reqs):
+                        time.sleep(max(0.0, min(0.05, reqs[nxt][1][0] - (time.perf_counter() - t0))))
+                    continue
             active = self._schedule(left, held)
             tb = time.perf_counter()
             self.batch.step(self.tick)
+            # while the tick runs: admit the waiting requests whose worlds
+            # are built (their uploads overlap the tick; they step from the
+            # next one)
+            self._admit(waiting, live, left, rejected, t0)
+            self._prefetch(waiting)
             fin = self.batch.finished()  # waits for the tick
             busy += time.perf_counter() - tb
             tnow = time.perf_counter() - t0
