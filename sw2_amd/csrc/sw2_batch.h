@@ -3,7 +3,8 @@
 // by the batch code of sw2_device.hip.  Internal to libsw2.
 //
 // Each queues one step's height and momentum kernels of a batch on stream s:
-// mem, st, hdr are the device member records, step states and header; sch is
+// mem and st are the device member records and step states, hdr the host copy
+// of the header (the kernels take it by value); sch is
 // SW2's limiter (0-9); hv is 0 (no hyperviscosity) or 2 (del-2 with del-4);
 // tr selects the kernels that carry the tracer's code; ge and gu are the
 // workgroups of the two kernels; pe and pu ask for persistent workgroups

@@ -31,7 +31,7 @@ namespace SW2_SHAPE_NS {
 #include "sw2_mixed.inc"
 
 template <int SCH, int HV, bool TR>
-static void phases(const BMember *mem, const BState *st, const BHdr *hdr, int ge, int gu, int pe, int pu, hipStream_t s) {
+static void phases(const BMember *mem, const BState *st, const BHdr &hdr, int ge, int gu, int pe, int pu, hipStream_t s) {
 #ifdef SW2_PERSISTENT
     if (pe) kbp_eta<SCH, HV, TR><<<ge, NTE, 0, s>>>(mem, st, hdr);
     else kb_eta<SCH, HV, TR><<<ge, NTE, 0, s>>>(mem, st, hdr);
@@ -46,7 +46,7 @@ static void phases(const BMember *mem, const BState *st, const BHdr *hdr, int ge
 }
 
 template <int SCH>
-static void phases_sch(const BMember *mem, const BState *st, const BHdr *hdr, int hv, int tr, int ge, int gu, int pe,
+static void phases_sch(const BMember *mem, const BState *st, const BHdr &hdr, int hv, int tr, int ge, int gu, int pe,
                        int pu, hipStream_t s) {
     if (tr) {
         if (hv) phases<SCH, 2, true>(mem, st, hdr, ge, gu, pe, pu, s);
@@ -61,7 +61,7 @@ static void phases_sch(const BMember *mem, const BState *st, const BHdr *hdr, in
 SW2_BATCH_LAUNCHER(SW2_BATCH_PHASES) {
     const BMember *m = (const BMember *)mem;
     const BState *b = (const BState *)st;
-    const BHdr *h = (const BHdr *)hdr;
+    const BHdr &h = *(const BHdr *)hdr;  // the kernels take it by value
     switch (sch) {
         case 0: SW2_SHAPE_NS::phases_sch<0>(m, b, h, hv, tr, ge, gu, pe, pu, s); break;
         case 1: SW2_SHAPE_NS::phases_sch<1>(m, b, h, hv, tr, ge, gu, pe, pu, s); break;

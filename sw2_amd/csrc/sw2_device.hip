@@ -699,6 +699,7 @@ struct sw2_session {
     double *Lu, *Lv, *LLu, *LLv, *LLLu, *LLLv, *ze, *di, *le, *lle, *llle, *psi, *xi;
     double *pinc, *pdu, *pdv;
     Flags *fl;
+    int fl_borrowed;  // fl lies in a batch's array of guard records (the batch frees it)
     hipStream_t st;
     // host scratch for perturbations: storm injections as sparse lists
     // (inj, uploaded to d_idx/d_inc/d_trv), vortices as whole fields
@@ -811,7 +812,9 @@ static void frot(float *h[3]) {
     h[0] = m2;
 }
 
-extern "C" int sw2_session_create(sw *SW, int form, sw2_session **out) {
+// fl_at: where the session's guard record lives when a batch owns it (a
+// slot of the batch's array); NULL: the session allocates its own
+static int session_create(sw *SW, int form, sw2_session **out, Flags *fl_at) {
     *out = NULL;
     const int force_phased = (form & SW2_PHASED) != 0;
     form &= ~SW2_PHASED;
@@ -900,7 +903,12 @@ extern "C" int sw2_session_create(sw *SW, int form, sw2_session **out) {
     }
     if (form == SW2_FORM_MIXED) {
         if (!rc) rc = mixed_setup(s, SW);
-        if (!rc && hipMalloc((void **)&s->fl, sizeof(Flags)) != hipSuccess) rc = -1;
+        if (fl_at) {
+            s->fl = fl_at;
+            s->fl_borrowed = 1;
+        } else if (!rc && hipMalloc((void **)&s->fl, sizeof(Flags)) != hipSuccess) {
+            rc = -1;
+        }
         if (!rc && hipMemset(s->fl, 0, sizeof(Flags)) != hipSuccess) rc = -1;
         if (!rc && hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) rc = -1;
         if (rc) {
@@ -962,6 +970,8 @@ extern "C" int sw2_session_create(sw *SW, int form, sw2_session **out) {
     *out = s;
     return 0;
 }
+
+extern "C" int sw2_session_create(sw *SW, int form, sw2_session **out) { return session_create(SW, form, out, NULL); }
 
 extern "C" int sw2_session_begin(sw2_session *s, int nstart, double t) {
     s->nstart = nstart;
@@ -1585,7 +1595,7 @@ extern "C" void sw2_session_destroy(sw2_session *s) {
     if (s->d_part) (void)hipFree(s->d_part);
     if (s->d_cflp) (void)hipFree(s->d_cflp);
     free(s->h_hbar);
-    if (s->fl) (void)hipFree(s->fl);
+    if (s->fl && !s->fl_borrowed) (void)hipFree(s->fl);
     if (s->st) (void)hipStreamDestroy(s->st);
     for (int k = 0; k < 8; k++) {
         for (int q = 0; q < 3; q++) free(s->h_dv[q][k]);
@@ -1622,10 +1632,10 @@ struct sw2_batch {
     sw2_session **sess;
     size_t *foot;
     BMember *h_mem;
-    BHdr h_hdr;
+    BHdr h_hdr;  // the kernels' header, passed by value
     BMember *d_mem;
     BState *d_st;
-    BHdr *d_hdr;
+    Flags *d_flags;  // the members' guard records, by slot
     hipStream_t st;
     hipGraph_t graph;
     hipGraphExec_t gexec;
@@ -1648,8 +1658,9 @@ struct sw2_batch {
     hipEvent_t stage_ev[2];
     int stage_i;
     int *paused;  // host copy of the members' paused flags
-    int *h_tmap, *d_tmap;  // member slot of every flat tile, host and device
-    int tmap_cap;          // entries of both
+    int2 *h_tmap, *d_tmap;  // every flat tile's member slot and tile coordinates, host and device
+    int tmap_cap;           // entries of both
+    int xshift_set;         // log2 of the XCD block size when set (SW2_XCD_BLOCK), else -1
     int shape;             // tile shape of the batch kernels: an index of BATCH_SHAPES
 };
 
@@ -1711,7 +1722,8 @@ extern "C" int sw2_batch_create(int form, int capacity, double hbm_budget_bytes,
     int rc = 0;
     if (hipMalloc((void **)&b->d_mem, sizeof(BMember) * capacity) != hipSuccess) rc = -1;
     if (!rc && hipMalloc((void **)&b->d_st, sizeof(BState) * capacity) != hipSuccess) rc = -1;
-    if (!rc && hipMalloc((void **)&b->d_hdr, sizeof(BHdr)) != hipSuccess) rc = -1;
+    if (!rc && hipMalloc((void **)&b->d_flags, sizeof(Flags) * capacity) != hipSuccess) rc = -1;
+    if (!rc && hipMemset(b->d_flags, 0, sizeof(Flags) * capacity) != hipSuccess) rc = -1;
     if (!rc && hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) rc = -1;
     if (!rc) {
         BState *z = (BState *)calloc(capacity, sizeof(BState));
@@ -1720,9 +1732,14 @@ extern "C" int sw2_batch_create(int form, int capacity, double hbm_budget_bytes,
         free(z);
     }
     b->h_hdr.nslots = capacity;
+    b->h_hdr.flags = b->d_flags;
     b->h_hdr.total_tiles = 0;
-    b->h_hdr.xblock = 256;  // tiles per XCD block: 64..1024 measured within 2% of each other on 32 x 16 tiles
-    if (const char *e = getenv("SW2_XCD_BLOCK")) b->h_hdr.xblock = atoi(e) > 0 ? atoi(e) : 256;
+    b->xshift_set = -1;
+    if (const char *e = getenv("SW2_XCD_BLOCK")) {  // a power of two, 8 .. 4096
+        const int v = atoi(e);
+        for (int k = 3; k <= 12; k++)
+            if (v == 1 << k) b->xshift_set = k;
+    }
     b->table_dirty = 1;
     if (rc) {
         sw2_set_error("batch allocation failed");
@@ -1743,11 +1760,18 @@ static void batch_retile(sw2_batch *b) {
         b->h_mem[m].tile0 = t;
         if (!b->paused[m]) t += b->h_mem[m].ntx * b->h_mem[m].nty;
     }
-    // a graph launching one workgroup per tile holds the old tile count: it
-    // is captured again at the next step (a replay may still be running, so
-    // it is not destroyed here)
-    if (t != b->h_hdr.total_tiles && (b->blocks_eta == 0 || b->blocks_uv == 0) && b->gexec) b->graph_stale = 1;
+    // the graph holds the old tile count (in its header, and in its grids
+    // when launching one workgroup per tile): it is captured again at the
+    // next step (a replay may still be running, so it is not destroyed here)
+    if (t != b->h_hdr.total_tiles && b->gexec) b->graph_stale = 1;
     b->h_hdr.total_tiles = t;
+    // XCD blocks of 256 tiles, fewer when the tiles make less than two
+    // rounds of eight such blocks, so that small batches are dealt too
+    // (measured, a lone 1024 x 512 member: 1.22 -> 1.10 x its solo step
+    // with 64-tile blocks; 64 to 2048 within 2% on large ones)
+    int xs = 8;
+    while (xs > 3 && t < (16 << xs)) xs--;
+    b->h_hdr.xshift = b->xshift_set >= 0 ? b->xshift_set : xs;
     b->table_dirty = 1;
 }
 
@@ -1769,7 +1793,8 @@ extern "C" int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id) {
         return 1;
     }
     sw2_session *s = NULL;
-    if (sw2_session_create(SW, b->form, &s)) return -1;
+    // the session's guard record is the slot's in the batch's array
+    if (session_create(SW, b->form, &s, b->d_flags + slot)) return -1;
     if (b->sch < 0) {
         b->sch = s->sch;
         b->hvmode = s->hvmode;
@@ -1781,7 +1806,6 @@ extern "C" int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id) {
     BMember &M = b->h_mem[slot];
     memset(&M, 0, sizeof M);
     M.g = s->g;
-    M.fl = s->fl;
     if (!s->d_cflp) {
         HIPCHECK(hipMalloc((void **)&s->d_cflp, sizeof(unsigned long long) * 2 * NPART));
         HIPCHECK(hipMemset(s->d_cflp, 0, sizeof(unsigned long long) * 2 * NPART));
@@ -1802,6 +1826,12 @@ extern "C" int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id) {
     }
     M.ntx = (SW->nx + BATCH_SHAPES[b->shape].tx - 1) / BATCH_SHAPES[b->shape].tx;
     M.nty = (SW->ny + BATCH_SHAPES[b->shape].ty - 1) / BATCH_SHAPES[b->shape].ty;
+    if (M.ntx > 0xffff || M.nty > 0x7fff) {  // the tile map packs a tile's column and row into 16 + 15 bits
+        sw2_session_destroy(s);
+        memset(&M, 0, sizeof M);
+        sw2_set_error("a member of %d x %d cells has more tiles per row or column than a batch numbers", SW->nx, SW->ny);
+        return -1;
+    }
     M.nstart = 0;
     M.nlast = nsteps;
     M.Dt = s->Dt;
@@ -1843,24 +1873,25 @@ static void batch_launch_step(sw2_batch *b) {
     // persistent workgroups exist for the 32 x 16 tiles only
     const int hv = b->hvmode ? 2 : 0, tr = b->tracers > 0;
     if (b->shape == 1)
-        sw2_batch_phases_64x8(b->d_mem, b->d_st, b->d_hdr, b->sch, hv, tr, ge, gu, 0, 0, b->st);
+        sw2_batch_phases_64x8(b->d_mem, b->d_st, &b->h_hdr, b->sch, hv, tr, ge, gu, 0, 0, b->st);
     else if (b->shape == 2)
-        sw2_batch_phases_64x16(b->d_mem, b->d_st, b->d_hdr, b->sch, hv, tr, ge, gu, 0, 0, b->st);
+        sw2_batch_phases_64x16(b->d_mem, b->d_st, &b->h_hdr, b->sch, hv, tr, ge, gu, 0, 0, b->st);
     else
-        sw2_batch_phases_32x16(b->d_mem, b->d_st, b->d_hdr, b->sch, hv, tr, ge, gu, b->blocks_eta != 0,
+        sw2_batch_phases_32x16(b->d_mem, b->d_st, &b->h_hdr, b->sch, hv, tr, ge, gu, b->blocks_eta != 0,
                                b->blocks_uv != 0, b->st);
-    kb_adv<<<(b->cap + 3) / 4, 256, 0, b->st>>>(b->d_mem, b->d_st, b->d_hdr);
+    kb_adv<<<(b->cap + 3) / 4, 256, 0, b->st>>>(b->d_mem, b->d_st, b->h_hdr);
 }
 
 static int batch_upload(sw2_batch *b) {
     if (!b->table_dirty) return 0;
     // the tile map: flat tile -> member slot, in the numbering batch_retile
     // made (held members and empty slots own no tiles).  It grows by
-    // doubling; the kernels read its address from the header.
+    // doubling; the kernels take its address in the header (a new address
+    // makes the graph stale).
     const int t = b->h_hdr.total_tiles;
     if (t > b->tmap_cap) {
         const int cap = t > 2 * b->tmap_cap ? t : 2 * b->tmap_cap;
-        int *nh = (int *)realloc(b->h_tmap, sizeof(int) * (size_t)cap);
+        int2 *nh = (int2 *)realloc(b->h_tmap, sizeof(int2) * (size_t)cap);
         if (!nh) {
             sw2_set_error("out of host memory for the tile map of %d tiles", cap);
             return -1;
@@ -1869,16 +1900,17 @@ static int batch_upload(sw2_batch *b) {
         HIPCHECK(hipStreamSynchronize(b->st));
         if (b->d_tmap) HIPCHECK(hipFree(b->d_tmap));
         b->d_tmap = NULL;
-        HIPCHECK(hipMalloc((void **)&b->d_tmap, sizeof(int) * (size_t)cap));
+        HIPCHECK(hipMalloc((void **)&b->d_tmap, sizeof(int2) * (size_t)cap));
         b->tmap_cap = cap;
     }
     for (int m = 0, k = 0; m < b->cap; m++)
         if (!b->paused[m])
-            for (int e = b->h_mem[m].ntx * b->h_mem[m].nty; e > 0; e--) b->h_tmap[k++] = m;
+            for (int ty = 0; ty < b->h_mem[m].nty; ty++)
+                for (int tx = 0; tx < b->h_mem[m].ntx; tx++) b->h_tmap[k++] = make_int2(m, tx | (ty << 16));
+    if (b->h_hdr.tmap != b->d_tmap && b->gexec) b->graph_stale = 1;
     b->h_hdr.tmap = b->d_tmap;
-    if (t > 0) HIPCHECK(hipMemcpyAsync(b->d_tmap, b->h_tmap, sizeof(int) * (size_t)t, hipMemcpyHostToDevice, b->st));
+    if (t > 0) HIPCHECK(hipMemcpyAsync(b->d_tmap, b->h_tmap, sizeof(int2) * (size_t)t, hipMemcpyHostToDevice, b->st));
     HIPCHECK(hipMemcpyAsync(b->d_mem, b->h_mem, sizeof(BMember) * b->cap, hipMemcpyHostToDevice, b->st));
-    HIPCHECK(hipMemcpyAsync(b->d_hdr, &b->h_hdr, sizeof(BHdr), hipMemcpyHostToDevice, b->st));
     HIPCHECK(hipStreamSynchronize(b->st));
     b->table_dirty = 0;
     return 0;
@@ -2284,7 +2316,7 @@ extern "C" void sw2_batch_destroy(sw2_batch *b) {
         if (b->sess && b->sess[m]) sw2_session_destroy(b->sess[m]);
     if (b->d_mem) (void)hipFree(b->d_mem);
     if (b->d_st) (void)hipFree(b->d_st);
-    if (b->d_hdr) (void)hipFree(b->d_hdr);
+    if (b->d_flags) (void)hipFree(b->d_flags);
     if (b->d_tmap) (void)hipFree(b->d_tmap);
     free(b->h_tmap);
     if (b->d_wt) (void)hipFree(b->d_wt);
