@@ -222,6 +222,51 @@ def test_all_members_held_then_released():
     b.close()
 
 
+def test_storm_member_failing_inside_a_replayed_window():
+    """A storm member whose storm injects a height beyond SW2's range check
+    (check_stats_scaf, sppde_core.c:377-395) at step 21, in the middle of
+    the batch's second replayed graph (16 steps each): the late height check
+    flags that step, the member stops there with its state of that step, and
+    a quiet member and a healthy storm member beside it run on.  Every
+    member's failed step, fields and storm marks equal its solo run's."""
+    def run_solo(p, bumps, n):
+        w, _ = cases.make_world(p, bumps)
+        s = Session(w, FORM_MIXED)
+        s.begin(0, p.t0)
+        s.step(n)
+        s.download()
+        return w, s.failed_step()[0]
+
+    n = 48
+    base = cases.scaled_params(256, 128, steps=n, lat=70.0)
+    t_on = base.t0 + 20.5 * base.Dt  # active from step 21 (t = t0 + 21 Dt), half a step in
+    G_bad = np.array([[t_on, t_on + 1e6, 1.0, 120.0, 20.0, 1, 0.0, 1e30, 1, 3.0, 8.0, 0.0]])
+    G_ok = np.array([[0.0, 1e6, 200.0, 300.0, -20.0, 1, 0.4, 3e12, 1, 3.0, 8.0, 0.0]])
+    specs = [("bad", dict(Gaussians=G_bad)), ("ok", dict(Gaussians=G_ok)), ("quiet", {})]
+    b = Batch(capacity=4)
+    b.set_launch(0, 0, 16)
+    members = []
+    for k, (name, kw) in enumerate(specs):
+        p = cases.scaled_params(256, 128, steps=n, lat=70.0, **kw)
+        bumps = cases.single_bump(7 + k)
+        w, _ = cases.make_world(p, bumps)
+        members.append((name, b.add(w, n), p, bumps))
+    b.step(n)
+    b.sync()
+    for name, mid, p, bumps in members:
+        ref, f_ref = run_solo(p, bumps, n)
+        st = b.member(mid)
+        assert st["done"] and st["failed_step"] == f_ref, (name, st, f_ref)
+        if name == "bad":
+            assert f_ref == 21, f_ref
+        else:
+            assert f_ref == 0 and st["steps"] == n, (name, st)
+        w = b.download(mid)
+        same(w, ref, f"{name} member")
+        assert np.array_equal(w.field("perturbs"), ref.field("perturbs")), name
+    b.close()
+
+
 @pytest.mark.parametrize("tiles", [(64, 8), (64, 16)])
 def test_tile_shapes_equal_solo_runs(tiles):
     """The batch kernels on taller tiles: every member (ragged sizes, storms,
