@@ -330,7 +330,10 @@ size_t sw2_batch_footprint(const sw *SW, int form);
 int sw2_batch_create(int form, int capacity, double hbm_budget_bytes, sw2_batch **out);
 /* Admit a world (its current state is the initial state) for nsteps steps.
  * Returns 0 and the member id, 1 when there is no room (slots or HBM
- * budget; nothing changed), or a negative error. */
+ * budget; nothing changed), or a negative error.  The world's upload
+ * completes before it returns; it does not wait for steps the batch still
+ * runs (the member joins at the next sw2_batch_step), so a server can admit
+ * while a tick runs. */
 int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id);
 /* Launch shape: workgroups of the height and momentum kernels and steps per
  * captured graph (the autotuner's knobs).  0 workgroups: one per tile of
