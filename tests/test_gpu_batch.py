@@ -104,6 +104,34 @@ def test_continuous_batching_join_and_leave():
     b.close()
 
 
+def test_members_admitted_while_steps_run():
+    """sw2_batch_add does not wait for the batch's queued steps: members
+    admitted right after a sw2_batch_step (a replayed graph still running,
+    or not yet started) join at the next step, and every member, early or
+    late, ends on its solo result."""
+    members = small_members(5, 6)
+    b = Batch(capacity=8)
+    b.set_launch(0, 0, 8)
+    ids = []
+    for p, bumps, steps in members[:3]:
+        w, _ = cases.make_world(p, bumps)
+        ids.append(b.add(w, steps))
+    b.step(16)  # returns at once; the graph runs while the next members are admitted
+    late = []
+    for p, bumps, steps in members[3:]:
+        w, _ = cases.make_world(p, bumps)
+        late.append(b.add(w, steps))
+    assert all(b.member(mid)["steps"] == 0 for mid in late)  # joined, not yet stepped
+    assert all(b.member(mid)["steps"] == 16 for mid in ids)
+    b.step(max(m[2] for m in members) + 5)
+    b.sync()
+    for mid, (p, bumps, steps) in zip(ids + late, members):
+        st = b.member(mid)
+        assert st["done"] and st["steps"] == steps and st["failed_step"] == 0, st
+        same(b.download(mid), solo(p, bumps, steps), f"member {mid}")
+    b.close()
+
+
 def test_admission_by_hbm_footprint():
     p = cases.member_params(512, 256, 10)
     w, _ = cases.make_world(p, [])
