@@ -2033,11 +2033,9 @@ extern "C" int sw2_batch_get_tiles(sw2_batch *b, int *tile_x, int *tile_y) {
     return 0;
 }
 
-// advance every active member by nsteps (members stop at their own last
-// step): whole graphs of gsteps steps, the remainder launched directly
-// after a batch step: the storms of every member that took it, into the
-// buffer holding its new height, and its height check after them (as a
-// single mixed session orders them).  The member's session counts its
+// after a directly launched batch step: the storms of every member that
+// took it, into the buffer holding its new height, and its tracer and
+// height checks after them (as a single mixed session orders them).  The member's session counts its
 // steps and time as a single session does (time by repeated addition of
 // Dt), so its storms are those of its own run.
 static int batch_storms(sw2_batch *b) {
