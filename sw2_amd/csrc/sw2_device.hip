@@ -516,6 +516,57 @@ __global__ __launch_bounds__(BX *BY) void k_check_mixed(Grid g, Flags *fl, MFiel
     if ((x < -1e10) || (x > 1e10) || (i == 1 && j == 1 && x != x)) flag_bad(fl, n, kind);
 }
 
+// a vortex's increments on a mixed form field (perturbs.c:397-460; eta' for
+// eta, whose mean does not change): value and residual of the interior
+// cells, their halo images by k_halo_mixed after every vortex of the step
+__global__ __launch_bounds__(BX *BY) void k_add_mixed(Grid g, Flags *fl, MField f, const double *inc) {
+    const int i = 1 + blockIdx.x * BX + threadIdx.x;
+    const int j = 1 + blockIdx.y * BY + threadIdx.y;
+    if (fl->fail_step || i > g.nx || j > g.ny) return;
+    const int q = IX(g, i, j);
+    f.hi[q] = mset_lo(f, q, mget(f, q) + inc[q]);
+}
+
+// halo_update (sppde_hupdate.c:274-363) on the values of a mixed form field,
+// as k_halo does on an FP64 field: every halo cell the periodic image
+__global__ void k_halo_mixed(Grid g, Flags *fl, float *f) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (fl->fail_step) return;
+    const int W = g.nx + 4;
+    int i, j;
+    if (t < 4 * W) {
+        const int r = t / W;
+        i = (t % W) - 1;
+        j = r < 2 ? r - 1 : g.ny + (r - 1);
+    } else {
+        const int t2 = t - 4 * W;
+        if (t2 >= 4 * g.ny) return;
+        j = 1 + t2 / 4;
+        const int c = t2 % 4;
+        i = c < 2 ? c - 1 : g.nx + (c - 1);
+    }
+    f[IX(g, i, j)] = f[IX(g, wrap(i, g.nx), wrap(j, g.ny))];
+}
+
+// do_channel (calc.c:600-649) on the mixed form's u and v, as k_channel:
+// interior columns only; the wall row ny of v is an interior row, so its
+// residual is cleared with its value
+__global__ void k_channel_mixed(Grid g, Flags *fl, MField u, MField v) {
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (fl->fail_step || i > g.nx) return;
+    const int ny = g.ny;
+    v.hi[IX(g, i, 0)] = 0.f;
+    v.hi[IX(g, i, -1)] = 0.f;
+    u.hi[IX(g, i, 0)] = u.hi[IX(g, i, 1)];
+    u.hi[IX(g, i, -1)] = u.hi[IX(g, i, 1)];
+    v.hi[IX(g, i, ny)] = 0.f;
+    v.lo[IX(g, i, ny)] = 0.f;
+    v.hi[IX(g, i, ny + 1)] = 0.f;
+    v.hi[IX(g, i, ny + 2)] = 0.f;
+    u.hi[IX(g, i, ny + 1)] = u.hi[IX(g, i, ny)];
+    u.hi[IX(g, i, ny + 2)] = u.hi[IX(g, i, ny)];
+}
+
 // Storms of batch members inside a replayed graph.  The host evaluates a
 // window of steps ahead (the cells of every storm at each step, as a single
 // session does step by step) and writes the window table; the graph's three
@@ -885,9 +936,9 @@ static int session_create(sw *SW, int form, sw2_session **out, Flags *fl_at) {
     // del-2 with del-4 forms both passes in the kernels' LDS; del-6, or del-2
     // alone, takes the Laplacians from the per-phase kernels (mode 3)
     s->hvmode = !s->hv ? 0 : ((SW->nu6 == 0 && !(SW->nu2 != 0 && SW->nu4 == 0)) ? 2 : 3);
-    if (form == SW2_FORM_MIXED && !(s->fused && s->hvmode != 3 && SW->num_vortices == 0)) {
-        sw2_set_error("the mixed-precision form needs a case the fused step takes, without vortices, del-6 or del-2 "
-                      "alone: a j-only wind, nx >= 128, ny >= 4");
+    if (form == SW2_FORM_MIXED && !(s->fused && s->hvmode != 3)) {
+        sw2_set_error("the mixed-precision form needs a case the fused step takes, without del-6 or del-2 alone: a "
+                      "j-only wind, nx >= 128, ny >= 4");
         free(tab);
         free(s);
         return -1;
@@ -1122,12 +1173,18 @@ static int host_perturbations_on(sw2_session *s, int n, double t, int *eta_chang
             HIPCHECK(hipMemcpyAsync(s->pinc, s->h_dv[0][k], sizeof(double) * S, hipMemcpyHostToDevice, st));
             HIPCHECK(hipMemcpyAsync(s->pdu, s->h_dv[1][k], sizeof(double) * S, hipMemcpyHostToDevice, st));
             HIPCHECK(hipMemcpyAsync(s->pdv, s->h_dv[2][k], sizeof(double) * S, hipMemcpyHostToDevice, st));
-            k_add<<<G, B, 0, st>>>(s->g, s->fl, s->eta, s->pinc);
-            k_add<<<G, B, 0, st>>>(s->g, s->fl, s->u, s->pdu);
-            k_add<<<G, B, 0, st>>>(s->g, s->fl, s->v, s->pdv);
-            launch_halo(s, s->eta);
-            launch_halo(s, s->u);
-            launch_halo(s, s->v);
+            if (s->form == SW2_FORM_MIXED) {  // the halo images follow, once, in step_mixed
+                k_add_mixed<<<G, B, 0, st>>>(s->g, s->fl, meta, s->pinc);
+                k_add_mixed<<<G, B, 0, st>>>(s->g, s->fl, s->m_u, s->pdu);
+                k_add_mixed<<<G, B, 0, st>>>(s->g, s->fl, s->m_v, s->pdv);
+            } else {
+                k_add<<<G, B, 0, st>>>(s->g, s->fl, s->eta, s->pinc);
+                k_add<<<G, B, 0, st>>>(s->g, s->fl, s->u, s->pdu);
+                k_add<<<G, B, 0, st>>>(s->g, s->fl, s->v, s->pdv);
+                launch_halo(s, s->eta);
+                launch_halo(s, s->u);
+                launch_halo(s, s->v);
+            }
             *eta_changed = 1;
             HIPCHECK(hipStreamSynchronize(st));  // host scratch is reused
         }
@@ -1257,7 +1314,7 @@ static int step_fused(sw2_session *s) {
 static int step_phased(sw2_session *s);
 
 template <int SCH, int HV, bool TR>
-static void launch_mixed(sw2_session *s, int n, int k, int slot) {
+static void launch_mixed(sw2_session *s, int n, int k, int slot, int late) {
     dim3 G((s->g.nx + FTX - 1) / FTX, (s->g.ny + FTY - 1) / FTY);
     MEtaArgs ea{s->m_eta, s->m_u, s->m_v, s->m_De[1], s->m_De[2], s->m_eta2, s->m_De[0], s->d_mt, n, k, slot,
                 s->Dt, 2.0 * s->Dt * (s->sigma), s->c2, s->c4, s->courant, s->rest_x, s->rest_y,
@@ -1266,7 +1323,7 @@ static void launch_mixed(sw2_session *s, int n, int k, int slot) {
     km_eta<SCH, HV, TR><<<G, NTE, 0, s->st>>>(s->g, s->fl, ea);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 1], s->st);
     MUVArgs ua{s->m_eta2, s->m_u, s->m_v, s->m_Du[1], s->m_Du[2], s->m_Dv[1], s->m_Dv[2], s->m_u2, s->m_v2, s->m_Du[0],
-               s->m_Dv[0], s->d_mt, s->d_cflp, n, k, s->Dt, s->sigma, s->c2, s->c4, s->SW->num_Gaussians > 0,
+               s->m_Dv[0], s->d_mt, s->d_cflp, n, k, s->Dt, s->sigma, s->c2, s->c4, late,
                s->m_tr, s->m_Dtr[0], s->m_Dtr[1], s->m_Dtr[2], s->tracer_active};
     km_uv<SCH, HV, TR><<<G, NTU, 0, s->st>>>(s->g, s->fl, ua);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 2], s->st);
@@ -1275,13 +1332,13 @@ static void launch_mixed(sw2_session *s, int n, int k, int slot) {
 
 // the kernels with the tracer's code only for a world whose tracer runs
 template <int SCH>
-static void launch_mixed_hv(sw2_session *s, int n, int k, int slot) {
+static void launch_mixed_hv(sw2_session *s, int n, int k, int slot, int late) {
     if (s->tracer_active) {
-        if (s->hvmode) launch_mixed<SCH, 2, true>(s, n, k, slot);
-        else launch_mixed<SCH, 0, true>(s, n, k, slot);
+        if (s->hvmode) launch_mixed<SCH, 2, true>(s, n, k, slot, late);
+        else launch_mixed<SCH, 0, true>(s, n, k, slot, late);
     } else {
-        if (s->hvmode) launch_mixed<SCH, 2, false>(s, n, k, slot);
-        else launch_mixed<SCH, 0, false>(s, n, k, slot);
+        if (s->hvmode) launch_mixed<SCH, 2, false>(s, n, k, slot, late);
+        else launch_mixed<SCH, 0, false>(s, n, k, slot, late);
     }
 }
 
@@ -1293,7 +1350,13 @@ static int step_mixed(sw2_session *s) {
         km_cfl<<<1024, 256, 0, s->st>>>(s->g, s->fl, s->m_u, s->m_v, slot);
         s->cfl_ready = 1;
     }
-    SCH_SWITCH(launch_mixed_hv, s, n, k, slot);
+    const double t = s->t + s->Dt;
+    // a vortex added after u and v (core.c:292-296) changes eta, u and v:
+    // their checks and the wall wait for it, and the next step's CFL maxima
+    // are reduced again from the new u and v (as on the fused FP64 step)
+    const int vortex = s->SW->num_vortices > 0 && vortex_fires(s->SW, s->Dt, t);
+    const int late = ((s->SW->num_Gaussians > 0 || vortex) ? MLATE_TE : 0) | (vortex ? MLATE_UV : 0);
+    SCH_SWITCH(launch_mixed_hv, s, n, k, slot, late);
     std::swap(s->m_eta, s->m_eta2);
     std::swap(s->m_u, s->m_u2);
     std::swap(s->m_v, s->m_v2);
@@ -1301,18 +1364,30 @@ static int step_mixed(sw2_session *s) {
     frot(s->m_Du);
     frot(s->m_Dv);
     if (s->tracer_active) frot(s->m_Dtr);
-    const double t = s->t + s->Dt;
-    if (s->SW->num_Gaussians) {
-        // storms after u and v, then the tracer and height checks (core.c:280-312)
-        if (s->perturbs_dirty) {
+    if (s->SW->num_Gaussians || vortex) {
+        // storms and vortices after u and v, then the checks (core.c:280-312)
+        if (s->perturbs_dirty && s->pert) {
             k_zero_pert<<<(s->S + 255) / 256, 256, 0, s->st>>>(s->fl, n, s->S, s->pert);
             s->perturbs_dirty = 0;
         }
         int eta_changed = 0;
         if (host_perturbations(s, n, t, &eta_changed)) return -1;
-        if (s->tracer_active)
-            k_check_mixed<<<cells_grid(s->g), cells_block(), 0, s->st>>>(s->g, s->fl, s->m_tr, nullptr, n, 2);
-        k_check_mixed<<<cells_grid(s->g), cells_block(), 0, s->st>>>(s->g, s->fl, s->m_eta, s->d_mt + MT_HBAR * s->g.R, n, 3);
+        const dim3 G = cells_grid(s->g), B = cells_block();
+        if (vortex) {  // the vortices' halo_update of eta, u, v, then do_channel (perturbs.c:433-434, core.c:298-306)
+            const int cnt = 4 * (s->g.nx + 4) + 4 * s->g.ny;
+            k_halo_mixed<<<(cnt + 255) / 256, 256, 0, s->st>>>(s->g, s->fl, s->m_eta.hi);
+            k_halo_mixed<<<(cnt + 255) / 256, 256, 0, s->st>>>(s->g, s->fl, s->m_u.hi);
+            k_halo_mixed<<<(cnt + 255) / 256, 256, 0, s->st>>>(s->g, s->fl, s->m_v.hi);
+            k_channel_mixed<<<(s->g.nx + 255) / 256, 256, 0, s->st>>>(s->g, s->fl, s->m_u, s->m_v);
+        }
+        if (s->tracer_active) k_check_mixed<<<G, B, 0, s->st>>>(s->g, s->fl, s->m_tr, nullptr, n, 2);
+        k_check_mixed<<<G, B, 0, s->st>>>(s->g, s->fl, s->m_eta, s->d_mt + MT_HBAR * s->g.R, n, 3);
+        if (vortex) {
+            k_check_mixed<<<G, B, 0, s->st>>>(s->g, s->fl, s->m_u, nullptr, n, 4);
+            k_check_mixed<<<G, B, 0, s->st>>>(s->g, s->fl, s->m_v, nullptr, n, 5);
+            k_cfl_clear<<<1, 1, 0, s->st>>>(s->fl, slot ^ 1);
+            km_cfl<<<1024, 256, 0, s->st>>>(s->g, s->fl, s->m_u, s->m_v, slot ^ 1);
+        }
     }
     s->n = n;
     s->t = t;
@@ -1791,6 +1866,10 @@ extern "C" int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id) {
     if (b->used + foot > b->budget) {
         sw2_set_error("HBM budget: %zu bytes in use, member needs %zu, budget %zu", b->used, foot, b->budget);
         return 1;
+    }
+    if (SW->num_vortices) {  // a vortex step needs its own late checks and wall: single sessions only
+        sw2_set_error("batches take cases without vortices (a mixed session runs them)");
+        return -1;
     }
     sw2_session *s = NULL;
     // the session's guard record is the slot's in the batch's array

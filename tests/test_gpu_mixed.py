@@ -127,13 +127,58 @@ def test_mixed_refuses_cases_the_fused_step_does_not_take():
     w, _ = cases.make_world(p, cases.single_bump(1))
     with pytest.raises(SW2Error, match="mixed-precision form"):
         Session(w, FORM_MIXED)
-    # vortices run on the FP64 steps only
-    V = np.array([[100.0, 200.0, -20.0, 8.0, 5.0, -60.0, 2.0]])
-    p = cases.scaled_params(128, 32, steps=5, lat=60.0, vortices=V)
+    # del-6 takes its Laplacians from the per-phase kernels: FP64 only
+    p = cases.scaled_params(128, 32, steps=5, lat=60.0, hypernu=(0.0, 3e14, 1e22))
     w, _ = cases.make_world(p, cases.single_bump(1))
     assert Session(w).fused
     with pytest.raises(SW2Error, match="mixed-precision form"):
         Session(w, FORM_MIXED)
+    # vortices run in a mixed session, not in a batch
+    from sw2_amd import Batch
+    V = np.array([[100.0, 200.0, -20.0, 8.0, 5.0, -60.0, 2.0]])
+    p = cases.scaled_params(128, 32, steps=5, lat=60.0, vortices=V)
+    w, _ = cases.make_world(p, cases.single_bump(1))
+    assert Session(w, FORM_MIXED).form == FORM_MIXED
+    b = Batch(capacity=2)
+    with pytest.raises(SW2Error, match="without vortices"):
+        b.add(w, 5)
+    b.close()
+
+
+def test_mixed_vortices_against_fp64():
+    """Vortices inserted during the run (perturbs.c:397-460) in the mixed
+    form: the increments added to eta', u', v' (value and residual) after
+    the momentum kernel, whose wall and u, v checks wait for them, then the
+    halo images, do_channel and the checks, and the next step's CFL maxima
+    reduced again, as the fused FP64 step orders them.  One vortex crosses
+    the periodic seam; one sits near the channel's southern half.  Against
+    the FP64 GPU run (equal to the oracle) within the tolerance, and the
+    vortices moved the flow."""
+    Vt = np.array([[100.0, 120.0, 20.0, 8.0, 5.0, -60.0, 2.0],
+                   [250.0, 356.0, -25.0, 6.0, 4.0, 40.0, 1.5]])
+    n = 200
+    p = cases.scaled_params(160, 64, steps=n, lat=75.0, vortices=Vt)
+    bumps = cases.single_bump(160 + 64)
+    runs = {}
+    for form in (FORM_FP64, FORM_MIXED):
+        w, _ = cases.make_world(p, bumps)
+        s = Session(w, form)
+        assert s.form == form and s.fused
+        s.begin(0, p.t0)
+        s.step(n)
+        s.download()
+        runs[form] = (w, s.failed_step()[0])
+    (w64, f64), (w32, f32) = runs[FORM_FP64], runs[FORM_MIXED]
+    assert f64 == f32 == 0
+    e = report(f"mixed vortices {n} steps", w32, w64.interior)
+    for f, (l2, mx) in e.items():
+        assert l2 <= TOL and mx <= TOL, (f, l2, mx)
+    w0, _ = cases.make_world(cases.scaled_params(160, 64, steps=n, lat=75.0), bumps)
+    s0 = Session(w0, FORM_FP64)
+    s0.begin(0, p.t0)
+    s0.step(n)
+    s0.download()
+    assert errors(w64.interior("v"), w0.interior("v"))[0] > 100 * TOL
 
 
 def test_mixed_tracer_storms_against_fp64():
