@@ -936,9 +936,13 @@ static int session_create(sw *SW, int form, sw2_session **out, Flags *fl_at) {
     // del-2 with del-4 forms both passes in the kernels' LDS; del-6, or del-2
     // alone, takes the Laplacians from the per-phase kernels (mode 3)
     s->hvmode = !s->hv ? 0 : ((SW->nu6 == 0 && !(SW->nu2 != 0 && SW->nu4 == 0)) ? 2 : 3);
-    if (form == SW2_FORM_MIXED && !(s->fused && s->hvmode != 3)) {
-        sw2_set_error("the mixed-precision form needs a case the fused step takes, without del-6 or del-2 alone: a "
-                      "j-only wind, nx >= 128, ny >= 4");
+    // the mixed kernels form del-2 and del-4 in LDS; del-2 alone runs there
+    // with a zero del-4 coefficient (the FP64 step takes it from the
+    // per-phase kernels only to leave SW2's sum without the del-4 term bit
+    // for bit); del-6 has no mixed form
+    if (form == SW2_FORM_MIXED && !(s->fused && SW->nu6 == 0)) {
+        sw2_set_error("the mixed-precision form needs a case the fused step takes, without del-6: a j-only wind, "
+                      "nx >= 128, ny >= 4");
         free(tab);
         free(s);
         return -1;

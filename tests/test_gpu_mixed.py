@@ -127,7 +127,7 @@ def test_mixed_refuses_cases_the_fused_step_does_not_take():
     w, _ = cases.make_world(p, cases.single_bump(1))
     with pytest.raises(SW2Error, match="mixed-precision form"):
         Session(w, FORM_MIXED)
-    # del-6 takes its Laplacians from the per-phase kernels: FP64 only
+    # del-6 has no mixed form (the FP64 step takes its Laplacians from the per-phase kernels)
     p = cases.scaled_params(128, 32, steps=5, lat=60.0, hypernu=(0.0, 3e14, 1e22))
     w, _ = cases.make_world(p, cases.single_bump(1))
     assert Session(w).fused
@@ -143,6 +143,30 @@ def test_mixed_refuses_cases_the_fused_step_does_not_take():
     with pytest.raises(SW2Error, match="without vortices"):
         b.add(w, 5)
     b.close()
+
+
+def test_mixed_del2_alone_against_fp64():
+    """Del-2 hyperviscosity without del-4: the FP64 step takes its Laplacian
+    from the per-phase kernels (SW2's sum carries no del-4 term at all); the
+    mixed kernels form it in LDS with a zero del-4 coefficient.  Within the
+    tolerance of the FP64 run."""
+    n = 200
+    p = cases.scaled_params(160, 64, steps=n, lat=75.0, hypernu=(2e5, 0.0, 0.0))
+    bumps = cases.single_bump(5)
+    runs = {}
+    for form in (FORM_FP64, FORM_MIXED):
+        w, _ = cases.make_world(p, bumps)
+        s = Session(w, form)
+        assert s.form == form and s.fused
+        s.begin(0, p.t0)
+        s.step(n)
+        s.download()
+        runs[form] = (w, s.failed_step()[0])
+    (w64, f64), (w32, f32) = runs[FORM_FP64], runs[FORM_MIXED]
+    assert f64 == f32 == 0
+    e = report(f"mixed del-2 alone {n} steps", w32, w64.interior)
+    for f, (l2, mx) in e.items():
+        assert l2 <= TOL and mx <= TOL, (f, l2, mx)
 
 
 def test_mixed_vortices_against_fp64():
