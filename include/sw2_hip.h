@@ -245,9 +245,9 @@ int load_binary_sw(int frame, int *nstep, double *timedouble, char *bname, char 
                               * residual per cell): FP32 fluxes, limiters, pressure
                               * gradients and Laplacians; FP64 mean terms, Adams-Bashforth
                               * accumulation, Coriolis update and reductions.  Cases the
-                              * fused step takes (sw2_session_fused) without del-6;
-                              * storms with or without tracer, and vortices (in a
-                              * session; batches take cases without vortices). */
+                              * fused step takes (sw2_session_fused): storms with or
+                              * without tracer, vortices and del-6 (in a session;
+                              * batches take cases without vortices or del-6). */
 /* Flag or'ed into the form: run one kernel per phase of the step even where
  * the fused two-kernel step applies (same results; for cross-checks). */
 #define SW2_PHASED 0x100

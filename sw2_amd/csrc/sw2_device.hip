@@ -527,6 +527,16 @@ __global__ __launch_bounds__(BX *BY) void k_add_mixed(Grid g, Flags *fl, MField 
     f.hi[q] = mset_lo(f, q, mget(f, q) + inc[q]);
 }
 
+// a mixed form field in FP64 (value + residual on the interior, the value
+// on the halo), the per-phase Laplacian kernels' input in mode 3
+__global__ void k_mixed_f64(Grid g, MField f, double *out, int S) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= S) return;
+    const int i = q % g.sx - 1, j = q / g.sx - 1;
+    const bool interior = i >= 1 && i <= g.nx && j >= 1 && j <= g.ny;
+    out[q] = interior ? mget(f, q) : (double)f.hi[q];
+}
+
 // halo_update (sppde_hupdate.c:274-363) on the values of a mixed form field,
 // as k_halo does on an FP64 field: every halo cell the periodic image
 __global__ void k_halo_mixed(Grid g, Flags *fl, float *f) {
@@ -736,6 +746,8 @@ struct sw2_session {
     float *m_De[3], *m_Du[3], *m_Dv[3];
     MField m_tr;      // the tracer (tracer_active), updated in place
     float *m_Dtr[3];  // and its tendencies
+    int mhv;          // the mixed kernels' hyperviscosity mode: 0, 2 (del-2/del-4 in LDS), 3 (del-6)
+    double *m_u64, *m_v64;  // mode 3: u, v in FP64, the per-phase Laplacian kernels' input
     double *d_mt, *h_hbar;
     double *d_part;          // energy reduction scratch
     unsigned long long *d_cflp;  // partial CFL maxima when stepped in a batch
@@ -832,6 +844,14 @@ static int mixed_setup(sw2_session *s, sw *SW) {
         rc |= falloc(&s->m_tr.hi, S);
         rc |= falloc(&s->m_tr.lo, S);
         for (int k = 0; k < 3; k++) rc |= falloc(&s->m_Dtr[k], S);
+    }
+    // del-6: the three Laplacian passes of u and v by the per-phase FP64
+    // kernels, from an FP64 copy of u and v, before each step
+    s->mhv = !s->hv ? 0 : (SW->nu6 != 0 ? 3 : 2);
+    if (s->mhv == 3) {
+        double **hvf[] = {&s->m_u64, &s->m_v64, &s->Lu, &s->Lv, &s->LLu, &s->LLv, &s->LLLu, &s->LLLv,
+                          &s->ze, &s->di, &s->le, &s->lle, &s->llle, &s->psi, &s->xi};
+        for (auto f : hvf) rc |= dalloc(f, S);
     }
     if (rc) return -1;
     // each field split into its FP32 value and residual
@@ -939,10 +959,10 @@ static int session_create(sw *SW, int form, sw2_session **out, Flags *fl_at) {
     // the mixed kernels form del-2 and del-4 in LDS; del-2 alone runs there
     // with a zero del-4 coefficient (the FP64 step takes it from the
     // per-phase kernels only to leave SW2's sum without the del-4 term bit
-    // for bit); del-6 has no mixed form
-    if (form == SW2_FORM_MIXED && !(s->fused && SW->nu6 == 0)) {
-        sw2_set_error("the mixed-precision form needs a case the fused step takes, without del-6: a j-only wind, "
-                      "nx >= 128, ny >= 4");
+    // for bit); del-6 takes its three Laplacian passes from the per-phase
+    // FP64 kernels, run on an FP64 copy of u and v (mixed_setup, step_mixed)
+    if (form == SW2_FORM_MIXED && !s->fused) {
+        sw2_set_error("the mixed-precision form needs a case the fused step takes: a j-only wind, nx >= 128, ny >= 4");
         free(tab);
         free(s);
         return -1;
@@ -1322,13 +1342,14 @@ static void launch_mixed(sw2_session *s, int n, int k, int slot, int late) {
     dim3 G((s->g.nx + FTX - 1) / FTX, (s->g.ny + FTY - 1) / FTY);
     MEtaArgs ea{s->m_eta, s->m_u, s->m_v, s->m_De[1], s->m_De[2], s->m_eta2, s->m_De[0], s->d_mt, n, k, slot,
                 s->Dt, 2.0 * s->Dt * (s->sigma), s->c2, s->c4, s->courant, s->rest_x, s->rest_y,
-                s->m_tr, s->m_Dtr[0], s->tracer_active, 2.0 * s->Dt * (s->tracer_dis)};
+                s->m_tr, s->m_Dtr[0], s->tracer_active, 2.0 * s->Dt * (s->tracer_dis), s->le, s->lle, s->llle, s->c6};
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i], s->st);
     km_eta<SCH, HV, TR><<<G, NTE, 0, s->st>>>(s->g, s->fl, ea);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 1], s->st);
     MUVArgs ua{s->m_eta2, s->m_u, s->m_v, s->m_Du[1], s->m_Du[2], s->m_Dv[1], s->m_Dv[2], s->m_u2, s->m_v2, s->m_Du[0],
                s->m_Dv[0], s->d_mt, s->d_cflp, n, k, s->Dt, s->sigma, s->c2, s->c4, late,
-               s->m_tr, s->m_Dtr[0], s->m_Dtr[1], s->m_Dtr[2], s->tracer_active};
+               s->m_tr, s->m_Dtr[0], s->m_Dtr[1], s->m_Dtr[2], s->tracer_active,
+               s->Lu, s->Lv, s->LLu, s->LLv, s->LLLu, s->LLLv, s->c6};
     km_uv<SCH, HV, TR><<<G, NTU, 0, s->st>>>(s->g, s->fl, ua);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 2], s->st);
     km_cfl_parts<<<1, 64, 0, s->st>>>(s->d_cflp, s->fl, slot ^ 1);
@@ -1338,10 +1359,12 @@ static void launch_mixed(sw2_session *s, int n, int k, int slot, int late) {
 template <int SCH>
 static void launch_mixed_hv(sw2_session *s, int n, int k, int slot, int late) {
     if (s->tracer_active) {
-        if (s->hvmode) launch_mixed<SCH, 2, true>(s, n, k, slot, late);
+        if (s->mhv == 3) launch_mixed<SCH, 3, true>(s, n, k, slot, late);
+        else if (s->mhv) launch_mixed<SCH, 2, true>(s, n, k, slot, late);
         else launch_mixed<SCH, 0, true>(s, n, k, slot, late);
     } else {
-        if (s->hvmode) launch_mixed<SCH, 2, false>(s, n, k, slot, late);
+        if (s->mhv == 3) launch_mixed<SCH, 3, false>(s, n, k, slot, late);
+        else if (s->mhv) launch_mixed<SCH, 2, false>(s, n, k, slot, late);
         else launch_mixed<SCH, 0, false>(s, n, k, slot, late);
     }
 }
@@ -1360,6 +1383,16 @@ static int step_mixed(sw2_session *s) {
     // are reduced again from the new u and v (as on the fused FP64 step)
     const int vortex = s->SW->num_vortices > 0 && vortex_fires(s->SW, s->Dt, t);
     const int late = ((s->SW->num_Gaussians > 0 || vortex) ? MLATE_TE : 0) | (vortex ? MLATE_UV : 0);
+    if (s->mhv == 3) {  // del-6: SW2's three passes of each Laplacian (calc.c:324-351, core.c:234-266)
+        k_mixed_f64<<<(s->S + 255) / 256, 256, 0, s->st>>>(s->g, s->m_u, s->m_u64, s->S);
+        k_mixed_f64<<<(s->S + 255) / 256, 256, 0, s->st>>>(s->g, s->m_v, s->m_v64, s->S);
+        lape(s, s->m_u64, s->le);
+        lape(s, s->le, s->lle);
+        lape(s, s->lle, s->llle);
+        lapv(s, s->m_u64, s->m_v64, s->Lu, s->Lv);
+        lapv(s, s->Lu, s->Lv, s->LLu, s->LLv);
+        lapv(s, s->LLu, s->LLv, s->LLLu, s->LLLv);
+    }
     SCH_SWITCH(launch_mixed_hv, s, n, k, slot, late);
     std::swap(s->m_eta, s->m_eta2);
     std::swap(s->m_u, s->m_u2);
@@ -1670,6 +1703,8 @@ extern "C" void sw2_session_destroy(sw2_session *s) {
         if (s->m_Dv[k]) (void)hipFree(s->m_Dv[k]);
         if (s->m_Dtr[k]) (void)hipFree(s->m_Dtr[k]);
     }
+    if (s->m_u64) (void)hipFree(s->m_u64);
+    if (s->m_v64) (void)hipFree(s->m_v64);
     if (s->d_mt) (void)hipFree(s->d_mt);
     if (s->d_part) (void)hipFree(s->d_part);
     if (s->d_cflp) (void)hipFree(s->d_cflp);
@@ -1871,8 +1906,9 @@ extern "C" int sw2_batch_add(sw2_batch *b, sw *SW, int nsteps, int *id) {
         sw2_set_error("HBM budget: %zu bytes in use, member needs %zu, budget %zu", b->used, foot, b->budget);
         return 1;
     }
-    if (SW->num_vortices) {  // a vortex step needs its own late checks and wall: single sessions only
-        sw2_set_error("batches take cases without vortices (a mixed session runs them)");
+    if (SW->num_vortices || SW->nu6 != 0) {  // a vortex step's late checks and wall, del-6's per-phase
+        // Laplacians: single sessions only
+        sw2_set_error("batches take cases without vortices or del-6 (a mixed session runs them)");
         return -1;
     }
     sw2_session *s = NULL;

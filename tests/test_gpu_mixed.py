@@ -127,22 +127,42 @@ def test_mixed_refuses_cases_the_fused_step_does_not_take():
     w, _ = cases.make_world(p, cases.single_bump(1))
     with pytest.raises(SW2Error, match="mixed-precision form"):
         Session(w, FORM_MIXED)
-    # del-6 has no mixed form (the FP64 step takes its Laplacians from the per-phase kernels)
-    p = cases.scaled_params(128, 32, steps=5, lat=60.0, hypernu=(0.0, 3e14, 1e22))
-    w, _ = cases.make_world(p, cases.single_bump(1))
-    assert Session(w).fused
-    with pytest.raises(SW2Error, match="mixed-precision form"):
-        Session(w, FORM_MIXED)
-    # vortices run in a mixed session, not in a batch
+    # vortices and del-6 run in a mixed session, not in a batch
     from sw2_amd import Batch
     V = np.array([[100.0, 200.0, -20.0, 8.0, 5.0, -60.0, 2.0]])
-    p = cases.scaled_params(128, 32, steps=5, lat=60.0, vortices=V)
-    w, _ = cases.make_world(p, cases.single_bump(1))
-    assert Session(w, FORM_MIXED).form == FORM_MIXED
-    b = Batch(capacity=2)
-    with pytest.raises(SW2Error, match="without vortices"):
-        b.add(w, 5)
-    b.close()
+    for kw in (dict(vortices=V), dict(hypernu=(0.0, 3e14, 1e22))):
+        p = cases.scaled_params(128, 32, steps=5, lat=60.0, **kw)
+        w, _ = cases.make_world(p, cases.single_bump(1))
+        assert Session(w, FORM_MIXED).form == FORM_MIXED
+        b = Batch(capacity=2)
+        with pytest.raises(SW2Error, match="without vortices or del-6"):
+            b.add(w, 5)
+        b.close()
+
+
+def test_mixed_del6_against_fp64():
+    """Del-2, del-4 and del-6 together: the mixed step takes the three
+    passes of both Laplacians (calc.c:324-351) from the per-phase FP64
+    kernels, run before the step on an FP64 copy of u and v, and adds SW2's
+    three terms in its order.  Within the tolerance of the FP64 run, whose
+    fused step takes them the same way."""
+    n = 100
+    p = cases.scaled_params(144, 56, steps=n, lat=75.0, hypernu=(2e5, 3e14, 1e22))
+    bumps = cases.single_bump(9)
+    runs = {}
+    for form in (FORM_FP64, FORM_MIXED):
+        w, _ = cases.make_world(p, bumps)
+        s = Session(w, form)
+        assert s.form == form and s.fused
+        s.begin(0, p.t0)
+        s.step(n)
+        s.download()
+        runs[form] = (w, s.failed_step()[0])
+    (w64, f64), (w32, f32) = runs[FORM_FP64], runs[FORM_MIXED]
+    assert f64 == f32 == 0
+    e = report(f"mixed del-6 {n} steps", w32, w64.interior)
+    for f, (l2, mx) in e.items():
+        assert l2 <= TOL and mx <= TOL, (f, l2, mx)
 
 
 def test_mixed_del2_alone_against_fp64():
