@@ -829,8 +829,6 @@ static int mixed_setup(sw2_session *s, sw *SW) {
         rc |= falloc(&f->hi, S);
         rc |= falloc(&f->lo, S);
     }
-    if (!rc && hipMalloc((void **)&s->d_cflp, sizeof(unsigned long long) * 2 * NPART) != hipSuccess) rc = -1;
-    if (!rc && hipMemset(s->d_cflp, 0, sizeof(unsigned long long) * 2 * NPART) != hipSuccess) rc = -1;
     if (!rc && SW->num_Gaussians) { /* the storms' marks, as SW2's perturbs field */
         rc = dalloc(&s->pert, S);
         if (!rc && hipMemcpy(s->pert, SW->perturbs, sizeof(double) * S, hipMemcpyHostToDevice) != hipSuccess) rc = -1;
@@ -972,6 +970,9 @@ static int session_create(sw *SW, int form, sw2_session **out, Flags *fl_at) {
     free(tab);
     s->g.T = s->d_tab;
     rc |= dalloc(&s->d_part, 3 * ENERGY_PARTS + 3);
+    // the momentum kernels' partial CFL maxima, zero between steps
+    if (!rc && hipMalloc((void **)&s->d_cflp, sizeof(unsigned long long) * 2 * NPART) != hipSuccess) rc = -1;
+    if (!rc && hipMemset(s->d_cflp, 0, sizeof(unsigned long long) * 2 * NPART) != hipSuccess) rc = -1;
     {
         double *sp = nullptr;  // the allocation-free energy's scratch address on this device
         if (!rc && scratch_address(&sp)) rc = -1;
@@ -1231,9 +1232,10 @@ static void launch_fused(sw2_session *s, int n, int k, int slot, int late) {
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 1], s->st);
     UVArgs ua{s->eta2, s->u, s->v, s->Du[1], s->Du[2], s->Dv[1], s->Dv[2], s->Dtr[0], s->Dtr[1], s->Dtr[2],
               s->u2, s->v2, s->Du[0], s->Dv[0], s->tr, n, k, slot ^ 1, late, s->Dt, s->sigma, s->c2, s->c4,
-              s->Lu, s->Lv, s->LLu, s->LLv, s->LLLu, s->LLLv, s->c6};
+              s->Lu, s->Lv, s->LLu, s->LLv, s->LLLu, s->LLLv, s->c6, s->d_cflp};
     kf_uv<SCH, HV><<<G, NTF, 0, s->st>>>(s->g, s->fl, ua);
     if (s->kev) (void)hipEventRecord(s->kev[3 * s->kev_i + 2], s->st);
+    km_cfl_parts<<<1, 64, 0, s->st>>>(s->d_cflp, s->fl, slot ^ 1);
 }
 
 template <int SCH>
